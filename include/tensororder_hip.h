@@ -1,0 +1,165 @@
+/*
+ * tensororder_hip.h -- C ABI of the MI355X (gfx950) contraction executor.
+ *
+ * The reference executes a contraction plan through a Python "tensor library" plugin
+ * (src/tensor_network/tensor_apis/base_api.py:8-28): per slice, a post-order walk that
+ * calls tensordot on every join (src/tensor_network/tensor_network.pyx:142-156,
+ * tensor_apis/numpy_apis.py:42-43), summing slice scalars in FP64 (base_api.py:17-28).
+ * This library replaces that arithmetic.  The host compiles a plan into a list of launch
+ * records whose descriptors live in device memory owned by the caller; every entry point
+ * takes plain pointers and sizes, never throws, and returns a TO_* status code.
+ *
+ * Tensors are dense all-dimension-2 arrays in C order: a rank-r tensor is a vector of 2^r
+ * entries and axis 0 is the most significant bit of the flat index.  A join contracts X
+ * and Y over s shared bits:
+ *     C[batch][(f << shf) | (y << shy)] =
+ *         sum_k X[batch*xb + pdep(f, xf_pos) + pdep(k, xs_pos)]
+ *             * Y[batch*yb + pdep(y, yf_pos) + pdep(k, ys_pos)]
+ * where pdep(v, pos) places bit i of v at bit position pos[i].  A batch stride of 0
+ * broadcasts an operand across the batch (slice-invariant tensors).
+ *
+ * Entry types: TO_F64 computes in float64 throughout.  TO_F32S computes in float32 and
+ * keeps, per tensor and batch item, a power-of-two exponent (to_scale_rec): true value =
+ * stored value * 2^exp; consumers rescale each operand by 2^-e, e = frexp exponent of the
+ * operand's max |entry|, so values never under- or overflow.  Slice results are summed in
+ * float64 as (mantissa, int64 exponent) pairs in both modes.
+ */
+#ifndef TENSORORDER_HIP_H
+#define TENSORORDER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TO_OK 0
+#define TO_ERR_OOM 1     /* caller should slice more: tensor_network.OutOfMemoryError */
+#define TO_ERR_INVALID 2 /* malformed launch record or descriptor */
+#define TO_ERR_HIP 3     /* HIP runtime error; see to_last_error() */
+
+/* entry types */
+#define TO_F64 0
+#define TO_F32S 1
+
+/* launch record op codes */
+#define TO_OP_LEAF_GATHER 1  /* descs: to_leaf_desc[count] */
+#define TO_OP_JOIN_STREAM 2  /* descs: to_join_desc[count]; variant = y values per thread */
+#define TO_OP_JOIN_SPLITK 3  /* descs: to_join_desc[count]; partial sums to d.partial */
+#define TO_OP_JOIN_FINISH 4  /* descs: to_join_desc[count]; sums d.partial into C */
+#define TO_OP_SCALE_CLEAR 5  /* aux0: to_scale_rec*, iaux0: record count */
+#define TO_OP_ROOT_REDUCE 6  /* aux0: root values, aux1: to_scale_rec* (F32S) or 0,
+                                aux2: to_accum*, aux3: wave params (int64[2]) */
+#define TO_OP_WAVE_ADVANCE 7 /* aux3: wave params; adds iaux0 to params[0] */
+#define TO_OP_STEP_BEGIN 8   /* aux2: to_accum* (zeroed), aux3: wave params (params[0] = 0) */
+
+#define TO_MAX_BITS 32
+
+/* One join of a launch group (a record in device memory). */
+typedef struct to_join_desc {
+    uint64_t x, y, c;          /* device pointers to X, Y, C (element type per entry type) */
+    int64_t xb, yb, cb;        /* batch strides in elements (0 = broadcast) */
+    uint64_t xs, ys, cs;       /* to_scale_rec* per batch item (TO_F32S), else 0 */
+    int64_t xsb, ysb;          /* scale-record batch strides (0 or 1) */
+    uint64_t partial;          /* split-K partial sums (TO_OP_JOIN_SPLITK / _FINISH) */
+    int32_t fx, fy, s;         /* X free bits, Y free bits, shared bits */
+    int32_t shf, shy;          /* output bit offset of f and of y */
+    int32_t nbatch;            /* batch items */
+    int32_t block0, nblocks;   /* blocks of this join within its launch */
+    int32_t nych;              /* y chunks (stream) */
+    int32_t nsplit;            /* k splits (split-K) */
+    int32_t pad0, pad1;
+    int8_t xf_pos[TO_MAX_BITS];
+    int8_t xs_pos[TO_MAX_BITS];
+    int8_t yf_pos[TO_MAX_BITS];
+    int8_t ys_pos[TO_MAX_BITS];
+} to_join_desc;
+
+/* One leaf copied per batch item from its table of slice rows. */
+typedef struct to_leaf_desc {
+    uint64_t src;          /* table: rows x 2^rank entries */
+    uint64_t dst;          /* nbatch x 2^rank entries */
+    uint64_t coef;         /* int64 [ngroups]: row = sum_g coef[g] * bit_g(slice) */
+    uint64_t rec;          /* to_scale_rec* per batch item (TO_F32S), else 0 */
+    int64_t exp0;          /* exponent of every stored entry (TO_F32S) */
+    int32_t rank, ngroups; /* ngroups = number of slice groups of the whole plan */
+    int32_t nbatch;
+    int32_t block0, nblocks;
+    int32_t pad;
+} to_leaf_desc;
+
+/* Per-tensor, per-batch-item scale (TO_F32S). */
+typedef struct to_scale_rec {
+    int64_t exp;      /* true value = stored value * 2^exp */
+    uint32_t maxbits; /* float bits of max |stored entry| (atomicMax on the bits) */
+    uint32_t pad;
+} to_scale_rec;
+
+/* Running FP64 sum of slice results: value = mant * 2^exp. */
+typedef struct to_accum {
+    double mant;
+    int64_t exp;
+    int64_t count; /* slice results accumulated */
+    int64_t pad;
+} to_accum;
+
+/* One kernel launch. */
+typedef struct to_launch {
+    int32_t op;
+    int32_t variant;
+    int32_t count;  /* descriptors */
+    int32_t grid;   /* blocks (256 threads each) */
+    int32_t lds;    /* dynamic LDS bytes */
+    int32_t pad;
+    uint64_t descs; /* device pointer to the descriptor array */
+    uint64_t aux0, aux1, aux2, aux3;
+    int64_t iaux0, iaux1;
+} to_launch;
+
+/* Library ABI version (bumped when a struct layout changes). */
+int to_version(void);
+
+/* Size in bytes of each record type, for host-side layout checks. */
+int to_struct_sizes(int64_t* join_desc, int64_t* leaf_desc, int64_t* scale_rec,
+                     int64_t* accum, int64_t* launch);
+
+/* Last error message of the calling thread ("" when none). */
+const char* to_last_error(void);
+
+/* Number of visible HIP devices. */
+int to_device_count(int* count);
+
+/* Select the device for subsequent calls on this thread. */
+int to_set_device(int device);
+
+/* Launch the records in order on `stream` (hipStream_t, 0 = default stream).
+ * Replaces the per-node tensordot calls of one post-order walk
+ * (src/tensor_network/tensor_network.pyx:142-156) for a whole batch of slices. */
+int to_run(const to_launch* launches, int count, int entry_type, void* stream);
+
+/* As to_run, recording a HIP event before each record and after the last on `stream`;
+ * waits for completion and stores each record's device time in ms[i] (count floats). */
+int to_run_timed(const to_launch* launches, int count, int entry_type, void* stream, float* ms);
+
+/* Capture the records into a HIP graph once; replay it with to_graph_launch. */
+int to_graph_create(const to_launch* launches, int count, int entry_type, void* stream,
+                    void** graph_exec);
+int to_graph_launch(void* graph_exec, void* stream);
+int to_graph_destroy(void* graph_exec);
+
+/* Reorder the axes of a rank-r all-dimension-2 tensor: out axis j = in axis perm[j],
+ * as numpy.transpose (used by tensor_apis/jax_apis.py:83-88, 146-162 and by
+ * tensor.py:112-117).  Element size 4 or 8 bytes. */
+int to_permute(const void* in, void* out, int rank, const int32_t* perm, int elem_bytes,
+               int64_t batch, void* stream);
+
+/* Block until all work on `stream` is done. */
+int to_synchronize(void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TENSORORDER_HIP_H */

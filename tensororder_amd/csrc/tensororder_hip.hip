@@ -1,0 +1,804 @@
+// MI355X (gfx950) contraction executor: kernels and the C ABI of include/tensororder_hip.h.
+//
+// Replaces the arithmetic of the reference's execution path: leaf materialisation per
+// slice (tensor_network_constructions.py:69-99, :144-152 via tensor.py:66-68), numpy
+// tensordot on every join of the post-order walk (tensor_network.pyx:142-156,
+// numpy_apis.py:42-43) and the FP64 sum of slice scalars (base_api.py:17-28).
+//
+// Design (CDNA4, wave64, 256-thread blocks):
+//  * every tensor is an all-dimension-2 array, so a join is an index-bit gather: the
+//    kernels never permute operands, they place bits with pdep tables instead;
+//  * the streamed operand X is read once, coalesced along its free bits; the other
+//    operand's slab (y chunk x 2^s) is staged in LDS and read as broadcasts;
+//  * small-output / large-shared joins split the shared range over blocks and finish
+//    with a deterministic fixed-order sum;
+//  * a batch of slices runs in one launch per join (batch stride 0 = shared tensor);
+//  * FP32 mode keeps one power-of-two exponent per tensor and batch item; consumers
+//    rescale on load, producers fold max |entry| into an atomicMax on its float bits.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "tensororder_hip.h"
+
+#define TO_ABI_VERSION 1
+
+namespace {
+
+constexpr int kBlock = 256;
+
+thread_local char g_err[512] = "";
+
+void set_err(const char* msg) {
+    snprintf(g_err, sizeof(g_err), "%s", msg);
+}
+
+int hip_fail(hipError_t e, const char* where) {
+    snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? TO_ERR_OOM : TO_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------------
+// device helpers
+
+__device__ __forceinline__ int64_t pdep(uint64_t v, const int8_t* pos, int n) {
+    int64_t r = 0;
+    for (int i = 0; i < n; ++i) r |= (int64_t)((v >> i) & 1ull) << pos[i];
+    return r;
+}
+
+template <class D>
+__device__ __forceinline__ int find_desc(const D* __restrict__ descs, int n, int bid) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (descs[mid].block0 <= bid) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// frexp exponent of a non-negative float given by its bits (0 for zero).
+__device__ __forceinline__ int scale_exp(uint32_t bits) {
+    const float m = __uint_as_float(bits);
+    if (!(m > 0.0f)) return 0;
+    int e;
+    frexpf(m, &e);
+    return e;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const T o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kBlock / 64; ++w) r += red[w];
+    }
+    return r;  // valid in thread 0
+}
+
+// Operand scale factor 2^-e for batch item b, and the exponent it contributes.
+struct OpScale {
+    float f;
+    int64_t e;
+};
+
+__device__ __forceinline__ OpScale op_scale(uint64_t recs, int64_t stride, int64_t b) {
+    const to_scale_rec* r = reinterpret_cast<const to_scale_rec*>(recs) + b * stride;
+    const int e = scale_exp(r->maxbits);
+    return OpScale{ldexpf(1.0f, -e), r->exp + e};
+}
+
+__device__ __forceinline__ void note_max(uint64_t recs, int64_t b, float m) {
+    to_scale_rec* r = reinterpret_cast<to_scale_rec*>(recs) + b;
+    atomicMax(&r->maxbits, __float_as_uint(m));
+}
+
+__device__ __forceinline__ void note_exp(uint64_t recs, int64_t b, int64_t e) {
+    to_scale_rec* r = reinterpret_cast<to_scale_rec*>(recs) + b;
+    r->exp = e;
+}
+
+template <typename T>
+__device__ __forceinline__ T absval(T v) {
+    return v < T(0) ? -v : v;
+}
+
+// ---------------------------------------------------------------------------------
+// leaf gather: per batch item, copy row sum_g coef[g]*bit_g(slice) of each leaf table
+
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_leaf_gather(const to_leaf_desc* __restrict__ descs,
+                                                        int ndesc, const int64_t* __restrict__ wave) {
+    const to_leaf_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int rank = d->rank;
+    const int64_t E = 1ll << rank;
+    const int64_t total = (int64_t)d->nbatch << rank;
+    const int64_t begin = wave ? wave[0] : 0;
+    const int G = d->ngroups;
+    const int64_t* coef = reinterpret_cast<const int64_t*>(d->coef);
+    const T* src = reinterpret_cast<const T*>(d->src);
+    T* dst = reinterpret_cast<T*>(d->dst);
+    const int64_t stride = (int64_t)d->nblocks * kBlock;
+    for (int64_t i = (int64_t)(blockIdx.x - d->block0) * kBlock + threadIdx.x; i < total; i += stride) {
+        const int64_t b = i >> rank;
+        const int64_t e = i & (E - 1);
+        const int64_t sidx = begin + b;
+        int64_t row = 0;
+        for (int g = 0; g < G; ++g) row += coef[g] * ((sidx >> (G - 1 - g)) & 1);
+        dst[i] = src[row * E + e];
+        if (SCALED && e == 0 && d->rec) {
+            to_scale_rec* r = reinterpret_cast<to_scale_rec*>(d->rec) + b;
+            r->exp = d->exp0;
+            r->maxbits = __float_as_uint(1.0f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// streaming join: each thread owns one free index f of X and YC free indices y of Y.
+// LDS: kx[2^s], ky[2^s] (uint32 offsets of the shared bits), then (fx >= 8 only) the
+// Y slab ylds[YC][2^s] of the current (batch item, y chunk).
+
+template <typename T, bool SCALED, int YC>
+__global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __restrict__ descs, int ndesc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int tid = threadIdx.x;
+    const int fx = d->fx, fy = d->fy, s = d->s;
+    const int K = 1 << s;
+    const int nych = d->nych;
+    const int nbatch = d->nbatch;
+    const T* __restrict__ X = reinterpret_cast<const T*>(d->x);
+    const T* __restrict__ Y = reinterpret_cast<const T*>(d->y);
+    T* __restrict__ C = reinterpret_cast<T*>(d->c);
+    const int64_t xb = d->xb, yb = d->yb, cb = d->cb;
+    const int shf = d->shf, shy = d->shy;
+    const int NY = 1 << fy;
+
+    uint32_t* kx = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* ky = kx + K;
+    T* ylds = reinterpret_cast<T*>(ky + K);
+    for (int k = tid; k < K; k += kBlock) {
+        kx[k] = (uint32_t)pdep(k, d->xs_pos, s);
+        ky[k] = (uint32_t)pdep(k, d->ys_pos, s);
+    }
+    __syncthreads();
+
+    const int local = blockIdx.x - d->block0;
+    const int nblk = d->nblocks;
+
+    if (fx >= 8) {
+        // tiles of 256 consecutive f inside one batch item; tile order puts y chunks
+        // innermost so a block revisits the same X rows while they are in cache
+        const int64_t nft = 1ll << (fx - 8);
+        const int64_t ntiles = (int64_t)nbatch * nft * nych;
+        const int64_t t0 = ntiles * local / nblk, t1 = ntiles * (local + 1) / nblk;
+        const int64_t xlo = pdep(tid, d->xf_pos, 8);
+        int64_t staged = -1;
+        for (int64_t t = t0; t < t1; ++t) {
+            const int ych = (int)(t % nych);
+            const int64_t bf = t / nych;
+            const int64_t ft = bf % nft;
+            const int64_t batch = bf / nft;
+            const int64_t key = batch * nych + ych;
+            const int ny = min(YC, NY - ych * YC);
+            float sx = 1.0f, sy = 1.0f;
+            int64_t ex = 0, ey = 0;
+            if (SCALED) {
+                const OpScale a = op_scale(d->xs, d->xsb, batch);
+                const OpScale c = op_scale(d->ys, d->ysb, batch);
+                sx = a.f; ex = a.e; sy = c.f; ey = c.e;
+            }
+            if (key != staged) {
+                __syncthreads();
+                const T* Yb = Y + batch * yb;
+                for (int i = tid; i < YC * K; i += kBlock) {
+                    const int yc = i >> s, k = i & (K - 1);
+                    T v = T(0);
+                    if (yc < ny) v = Yb[pdep(ych * YC + yc, d->yf_pos, fy) + ky[k]] * T(sy);
+                    ylds[i] = v;
+                }
+                __syncthreads();
+                staged = key;
+            }
+            const int64_t f = (ft << 8) | tid;
+            const T* Xp = X + batch * xb + pdep(ft, d->xf_pos + 8, fx - 8) + xlo;
+            T acc[YC];
+#pragma unroll
+            for (int yc = 0; yc < YC; ++yc) acc[yc] = T(0);
+            for (int k0 = 0; k0 < K; k0 += 8) {
+                T xv[8];
+#pragma unroll
+                for (int kk = 0; kk < 8; ++kk) xv[kk] = (k0 + kk < K) ? Xp[kx[k0 + kk]] * T(sx) : T(0);
+#pragma unroll
+                for (int yc = 0; yc < YC; ++yc) {
+#pragma unroll
+                    for (int kk = 0; kk < 8; ++kk) {
+                        if (k0 + kk < K) acc[yc] += xv[kk] * ylds[yc * K + k0 + kk];
+                    }
+                }
+            }
+            T* Cb = C + batch * cb;
+            float m = 0.0f;
+#pragma unroll
+            for (int yc = 0; yc < YC; ++yc) {
+                if (yc < ny) {
+                    const int64_t y = (int64_t)ych * YC + yc;
+                    Cb[(f << shf) | (y << shy)] = acc[yc];
+                    if (SCALED) m = fmaxf(m, (float)absval(acc[yc]));
+                }
+            }
+            if (SCALED) {
+                m = wave_max(m);
+                if ((tid & 63) == 0) note_max(d->cs, batch, m);
+                if (tid == 0 && ft == 0 && ych == 0) note_exp(d->cs, batch, ex + ey);
+            }
+        }
+    } else {
+        // tiles of 256 consecutive (batch item, f) pairs; Y read through the cache
+        const int fmask = (1 << fx) - 1;
+        const int64_t nL = (int64_t)nbatch << fx;
+        const int64_t nLt = (nL + kBlock - 1) / kBlock;
+        const int64_t ntiles = nLt * nych;
+        const int64_t t0 = ntiles * local / nblk, t1 = ntiles * (local + 1) / nblk;
+        const int64_t xoff = pdep(tid & fmask, d->xf_pos, fx);
+        const int f = tid & fmask;
+        for (int64_t t = t0; t < t1; ++t) {
+            const int ych = (int)(t % nych);
+            const int64_t lt = t / nych;
+            const int64_t L = lt * kBlock + tid;
+            if (L >= nL) continue;
+            const int64_t batch = L >> fx;
+            const int ny = min(YC, NY - ych * YC);
+            float sx = 1.0f, sy = 1.0f;
+            int64_t ex = 0, ey = 0;
+            if (SCALED) {
+                const OpScale a = op_scale(d->xs, d->xsb, batch);
+                const OpScale c = op_scale(d->ys, d->ysb, batch);
+                sx = a.f; ex = a.e; sy = c.f; ey = c.e;
+            }
+            const T* Xp = X + batch * xb + xoff;
+            const T* Yb = Y + batch * yb;
+            int64_t yoff[YC];
+#pragma unroll
+            for (int yc = 0; yc < YC; ++yc) yoff[yc] = (yc < ny) ? pdep(ych * YC + yc, d->yf_pos, fy) : 0;
+            T acc[YC];
+#pragma unroll
+            for (int yc = 0; yc < YC; ++yc) acc[yc] = T(0);
+            for (int k = 0; k < K; ++k) {
+                const T xv = Xp[kx[k]] * T(sx);
+#pragma unroll
+                for (int yc = 0; yc < YC; ++yc) {
+                    if (yc < ny) acc[yc] += xv * (Yb[yoff[yc] + ky[k]] * T(sy));
+                }
+            }
+            T* Cb = C + batch * cb;
+#pragma unroll
+            for (int yc = 0; yc < YC; ++yc) {
+                if (yc < ny) {
+                    const int64_t y = (int64_t)ych * YC + yc;
+                    Cb[((int64_t)f << shf) | (y << shy)] = acc[yc];
+                    if (SCALED) note_max(d->cs, batch, (float)absval(acc[yc]));
+                }
+            }
+            if (SCALED && f == 0 && ych == 0) note_exp(d->cs, batch, ex + ey);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// split-K join: block (o, sp) sums its share of the 2^s shared indices of output o.
+
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __restrict__ descs, int ndesc) {
+    __shared__ T red[kBlock / 64];
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int tid = threadIdx.x;
+    const int fx = d->fx, fy = d->fy, s = d->s;
+    const int nsplit = d->nsplit;
+    const int64_t local = blockIdx.x - d->block0;
+    const int64_t o = local / nsplit;
+    const int sp = (int)(local % nsplit);
+    const int64_t batch = o >> (fx + fy);
+    const int64_t f = (o >> fy) & ((1ll << fx) - 1);
+    const int64_t y = o & ((1ll << fy) - 1);
+    const T* X = reinterpret_cast<const T*>(d->x) + batch * d->xb + pdep(f, d->xf_pos, fx);
+    const T* Y = reinterpret_cast<const T*>(d->y) + batch * d->yb + pdep(y, d->yf_pos, fy);
+    float sx = 1.0f, sy = 1.0f;
+    if (SCALED) {
+        sx = op_scale(d->xs, d->xsb, batch).f;
+        sy = op_scale(d->ys, d->ysb, batch).f;
+    }
+    T sum = T(0);
+    if (s >= 8) {
+        const int64_t chunk_hi = (1ll << (s - 8)) / nsplit;  // 256-wide steps per split
+        const int64_t xlo = pdep(tid, d->xs_pos, 8), ylo = pdep(tid, d->ys_pos, 8);
+        for (int64_t kh = sp * chunk_hi; kh < (sp + 1) * chunk_hi; ++kh) {
+            const int64_t xo = pdep(kh, d->xs_pos + 8, s - 8) + xlo;
+            const int64_t yo = pdep(kh, d->ys_pos + 8, s - 8) + ylo;
+            sum += (X[xo] * T(sx)) * (Y[yo] * T(sy));
+        }
+    } else if (tid < (1 << s)) {
+        sum = (X[pdep(tid, d->xs_pos, s)] * T(sx)) * (Y[pdep(tid, d->ys_pos, s)] * T(sy));
+    }
+    sum = block_sum(sum, red);
+    if (tid == 0) reinterpret_cast<T*>(d->partial)[o * nsplit + sp] = sum;
+}
+
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_join_finish(const to_join_desc* __restrict__ descs, int ndesc) {
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int fx = d->fx, fy = d->fy;
+    const int nsplit = d->nsplit;
+    const int64_t nout = (int64_t)d->nbatch << (fx + fy);
+    const T* partial = reinterpret_cast<const T*>(d->partial);
+    T* C = reinterpret_cast<T*>(d->c);
+    const int64_t stride = (int64_t)d->nblocks * kBlock;
+    for (int64_t o = (int64_t)(blockIdx.x - d->block0) * kBlock + threadIdx.x; o < nout; o += stride) {
+        T sum = T(0);
+        for (int sp = 0; sp < nsplit; ++sp) sum += partial[o * nsplit + sp];
+        const int64_t batch = o >> (fx + fy);
+        const int64_t f = (o >> fy) & ((1ll << fx) - 1);
+        const int64_t y = o & ((1ll << fy) - 1);
+        C[batch * d->cb + ((f << d->shf) | (y << d->shy))] = sum;
+        if (SCALED) {
+            note_max(d->cs, batch, (float)absval(sum));
+            if (f == 0 && y == 0) {
+                const int64_t e = op_scale(d->xs, d->xsb, batch).e + op_scale(d->ys, d->ysb, batch).e;
+                note_exp(d->cs, batch, e);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// bookkeeping kernels
+
+__global__ __launch_bounds__(kBlock) void k_scale_clear(to_scale_rec* recs, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        recs[i].exp = 0;
+        recs[i].maxbits = 0u;
+        recs[i].pad = 0u;
+    }
+}
+
+__global__ void k_wave_advance(int64_t* wave, int64_t step) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) wave[0] += step;
+}
+
+__global__ void k_step_begin(to_accum* acc, int64_t* wave) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        acc->mant = 0.0;
+        acc->exp = 0;
+        acc->count = 0;
+        acc->pad = 0;
+        wave[0] = 0;
+    }
+}
+
+// Sum the root scalars of the valid batch items (slice index < wave[1]) into acc as
+// (mantissa, exponent): align every item to the largest exponent, add in a fixed order.
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_root_reduce(const T* __restrict__ roots,
+                                                        const to_scale_rec* __restrict__ recs,
+                                                        to_accum* acc, const int64_t* __restrict__ wave,
+                                                        int nbatch) {
+    __shared__ int64_t emax_s[kBlock / 64];
+    __shared__ double red[kBlock / 64];
+    const int tid = threadIdx.x;
+    const int64_t begin = wave[0], end = wave[1];
+    int64_t nvalid = end - begin;
+    nvalid = nvalid < 0 ? 0 : (nvalid > nbatch ? nbatch : nvalid);
+    int64_t emax = INT64_MIN;
+    for (int64_t b = tid; b < nvalid; b += kBlock) {
+        const double v = (double)roots[b];
+        if (v != 0.0) {
+            int e;
+            frexp(v, &e);
+            const int64_t ee = e + (SCALED ? recs[b].exp : 0);
+            emax = ee > emax ? ee : emax;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const int64_t o = __shfl_xor(emax, off, 64);
+        emax = o > emax ? o : emax;
+    }
+    if ((tid & 63) == 0) emax_s[tid >> 6] = emax;
+    __syncthreads();
+    emax = emax_s[0];
+    for (int w = 1; w < kBlock / 64; ++w) emax = emax_s[w] > emax ? emax_s[w] : emax;
+    double part = 0.0;
+    if (emax != INT64_MIN) {
+        for (int64_t b = tid; b < nvalid; b += kBlock) {
+            const double v = (double)roots[b];
+            if (v != 0.0) {
+                int e;
+                const double m = frexp(v, &e);
+                int64_t sh = e + (SCALED ? recs[b].exp : 0) - emax;
+                sh = sh < -2000 ? -2000 : sh;
+                part += ldexp(m, (int)sh);
+            }
+        }
+    }
+    const double total = block_sum(part, red);
+    if (tid == 0) {
+        double am = acc->mant;
+        int64_t ae = acc->exp;
+        if (emax != INT64_MIN && total != 0.0) {
+            if (am == 0.0) {
+                am = total;
+                ae = emax;
+            } else {
+                const int64_t E = ae > emax ? ae : emax;
+                int64_t d1 = ae - E, d2 = emax - E;
+                d1 = d1 < -2000 ? -2000 : d1;
+                d2 = d2 < -2000 ? -2000 : d2;
+                am = ldexp(am, (int)d1) + ldexp(total, (int)d2);
+                ae = E;
+            }
+            int e2;
+            const double m2 = frexp(am, &e2);
+            am = m2;
+            ae += e2;
+        }
+        acc->mant = am;
+        acc->exp = am == 0.0 ? 0 : ae;
+        acc->count += nvalid;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// bit-permutation of all-dimension-2 tensors, LDS-tiled: a tile spans the union U of
+// the m lowest input bits and the input bits that land on the m lowest output bits,
+// so the tile is read with the input's low bits contiguous and written with the
+// output's low bits contiguous.
+
+struct PermPlan {
+    int rank, nu, nrest, pad;
+    int8_t u_in[16];     // input bit position of tile bit j (u order: input-ascending)
+    int8_t v_out[16];    // output bit position of tile bit j (v order: output-ascending)
+    int8_t v2u[16];      // tile bit j of v order is tile bit v2u[j] of u order
+    int8_t rest_in[32];  // input position of rest bit j
+    int8_t rest_out[32]; // output position of rest bit j
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute(const T* __restrict__ in, T* __restrict__ out,
+                                                    PermPlan p, int64_t batch) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* tile = reinterpret_cast<T*>(smem);
+    const int U = 1 << p.nu;
+    const int64_t n = 1ll << p.rank;
+    const int64_t ntiles = batch << p.nrest;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t b = t >> p.nrest;
+        const int64_t r = t & ((1ll << p.nrest) - 1);
+        const int64_t ibase = b * n + pdep(r, p.rest_in, p.nrest);
+        const int64_t obase = b * n + pdep(r, p.rest_out, p.nrest);
+        __syncthreads();
+        for (int u = threadIdx.x; u < U; u += kBlock) tile[u] = in[ibase + pdep(u, p.u_in, p.nu)];
+        __syncthreads();
+        for (int v = threadIdx.x; v < U; v += kBlock) {
+            int u = 0;
+            for (int j = 0; j < p.nu; ++j) u |= ((v >> j) & 1) << p.v2u[j];
+            out[obase + pdep(v, p.v_out, p.nu)] = tile[u];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// host dispatch
+
+int grid_ok(const to_launch& L) {
+    if (L.grid <= 0 || L.lds < 0 || L.lds > 160 * 1024) return 0;
+    return 1;
+}
+
+template <typename T, bool SCALED>
+int launch_one(const to_launch& L, hipStream_t st) {
+    const dim3 grid(L.grid), block(kBlock);
+    switch (L.op) {
+        case TO_OP_LEAF_GATHER:
+            if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
+            hipLaunchKernelGGL((k_leaf_gather<T, SCALED>), grid, block, 0, st,
+                               reinterpret_cast<const to_leaf_desc*>(L.descs), L.count,
+                               reinterpret_cast<const int64_t*>(L.aux3));
+            break;
+        case TO_OP_JOIN_STREAM: {
+            if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
+            const to_join_desc* d = reinterpret_cast<const to_join_desc*>(L.descs);
+            switch (L.variant) {
+#define TO_STREAM_CASE(YC)                                                                     \
+    case YC:                                                                                   \
+        hipLaunchKernelGGL((k_join_stream<T, SCALED, YC>), grid, block, L.lds, st, d, L.count); \
+        break;
+                TO_STREAM_CASE(1)
+                TO_STREAM_CASE(2)
+                TO_STREAM_CASE(4)
+                TO_STREAM_CASE(8)
+                TO_STREAM_CASE(16)
+#undef TO_STREAM_CASE
+                default:
+                    set_err("stream join: unsupported y chunk");
+                    return TO_ERR_INVALID;
+            }
+            break;
+        }
+        case TO_OP_JOIN_SPLITK:
+            if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
+            hipLaunchKernelGGL((k_join_splitk<T, SCALED>), grid, block, 0, st,
+                               reinterpret_cast<const to_join_desc*>(L.descs), L.count);
+            break;
+        case TO_OP_JOIN_FINISH:
+            if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
+            hipLaunchKernelGGL((k_join_finish<T, SCALED>), grid, block, 0, st,
+                               reinterpret_cast<const to_join_desc*>(L.descs), L.count);
+            break;
+        case TO_OP_SCALE_CLEAR:
+            if (!L.aux0 || L.iaux0 <= 0) return TO_ERR_INVALID;
+            hipLaunchKernelGGL(k_scale_clear, grid, block, 0, st, reinterpret_cast<to_scale_rec*>(L.aux0),
+                               L.iaux0);
+            break;
+        case TO_OP_ROOT_REDUCE:
+            if (!L.aux0 || !L.aux2 || !L.aux3 || L.iaux0 <= 0) return TO_ERR_INVALID;
+            if (SCALED && !L.aux1) return TO_ERR_INVALID;
+            hipLaunchKernelGGL((k_root_reduce<T, SCALED>), dim3(1), block, 0, st,
+                               reinterpret_cast<const T*>(L.aux0),
+                               reinterpret_cast<const to_scale_rec*>(L.aux1),
+                               reinterpret_cast<to_accum*>(L.aux2), reinterpret_cast<const int64_t*>(L.aux3),
+                               (int)L.iaux0);
+            break;
+        case TO_OP_WAVE_ADVANCE:
+            if (!L.aux3) return TO_ERR_INVALID;
+            hipLaunchKernelGGL(k_wave_advance, dim3(1), dim3(64), 0, st, reinterpret_cast<int64_t*>(L.aux3),
+                               L.iaux0);
+            break;
+        case TO_OP_STEP_BEGIN:
+            if (!L.aux2 || !L.aux3) return TO_ERR_INVALID;
+            hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, st, reinterpret_cast<to_accum*>(L.aux2),
+                               reinterpret_cast<int64_t*>(L.aux3));
+            break;
+        default:
+            set_err("unknown launch op");
+            return TO_ERR_INVALID;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return TO_OK;
+}
+
+int run_records(const to_launch* launches, int count, int entry_type, hipStream_t st) {
+    if (!launches || count < 0) {
+        set_err("null launch list");
+        return TO_ERR_INVALID;
+    }
+    if (entry_type != TO_F64 && entry_type != TO_F32S) {
+        set_err("unknown entry type");
+        return TO_ERR_INVALID;
+    }
+    for (int i = 0; i < count; ++i) {
+        const to_launch& L = launches[i];
+        if (!grid_ok(L)) {
+            set_err("bad grid or LDS size in launch record");
+            return TO_ERR_INVALID;
+        }
+        const int rc = entry_type == TO_F64 ? launch_one<double, false>(L, st) : launch_one<float, true>(L, st);
+        if (rc != TO_OK) {
+            if (rc == TO_ERR_INVALID && g_err[0] == 0) set_err("invalid launch record");
+            return rc;
+        }
+    }
+    return TO_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int to_version(void) { return TO_ABI_VERSION; }
+
+int to_struct_sizes(int64_t* join_desc, int64_t* leaf_desc, int64_t* scale_rec, int64_t* accum,
+                    int64_t* launch) {
+    if (join_desc) *join_desc = sizeof(to_join_desc);
+    if (leaf_desc) *leaf_desc = sizeof(to_leaf_desc);
+    if (scale_rec) *scale_rec = sizeof(to_scale_rec);
+    if (accum) *accum = sizeof(to_accum);
+    if (launch) *launch = sizeof(to_launch);
+    return TO_OK;
+}
+
+const char* to_last_error(void) { return g_err; }
+
+int to_device_count(int* count) {
+    if (!count) return TO_ERR_INVALID;
+    const hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    return TO_OK;
+}
+
+int to_set_device(int device) {
+    const hipError_t e = hipSetDevice(device);
+    return e == hipSuccess ? TO_OK : hip_fail(e, "hipSetDevice");
+}
+
+int to_run(const to_launch* launches, int count, int entry_type, void* stream) {
+    g_err[0] = 0;
+    return run_records(launches, count, entry_type, reinterpret_cast<hipStream_t>(stream));
+}
+
+int to_run_timed(const to_launch* launches, int count, int entry_type, void* stream, float* ms) {
+    g_err[0] = 0;
+    if (!launches || count <= 0 || !ms) {
+        set_err("to_run_timed: invalid argument");
+        return TO_ERR_INVALID;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipEvent_t* ev = new hipEvent_t[count + 1];
+    int created = 0, rc = TO_OK;
+    for (; created <= count; ++created) {
+        const hipError_t e = hipEventCreate(&ev[created]);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "hipEventCreate");
+            break;
+        }
+    }
+    if (rc == TO_OK) {
+        for (int i = 0; i < count && rc == TO_OK; ++i) {
+            hipEventRecord(ev[i], st);
+            rc = run_records(launches + i, 1, entry_type, st);
+        }
+        hipEventRecord(ev[count], st);
+        const hipError_t e = hipEventSynchronize(ev[count]);
+        if (rc == TO_OK && e != hipSuccess) rc = hip_fail(e, "hipEventSynchronize");
+        for (int i = 0; i < count && rc == TO_OK; ++i) {
+            float t = 0.0f;
+            hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+            ms[i] = t;
+        }
+    }
+    for (int i = 0; i < created; ++i) hipEventDestroy(ev[i]);
+    delete[] ev;
+    return rc;
+}
+
+int to_graph_create(const to_launch* launches, int count, int entry_type, void* stream, void** graph_exec) {
+    g_err[0] = 0;
+    if (!graph_exec || !stream) {
+        set_err("graph capture needs a non-default stream");
+        return TO_ERR_INVALID;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    const int rc = run_records(launches, count, entry_type, st);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(st, &graph);
+    if (rc != TO_OK) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    *graph_exec = reinterpret_cast<void*>(exec);
+    return TO_OK;
+}
+
+int to_graph_launch(void* graph_exec, void* stream) {
+    g_err[0] = 0;
+    if (!graph_exec) return TO_ERR_INVALID;
+    const hipError_t e =
+        hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(graph_exec), reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? TO_OK : hip_fail(e, "hipGraphLaunch");
+}
+
+int to_graph_destroy(void* graph_exec) {
+    if (!graph_exec) return TO_OK;
+    const hipError_t e = hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(graph_exec));
+    return e == hipSuccess ? TO_OK : hip_fail(e, "hipGraphExecDestroy");
+}
+
+int to_permute(const void* in, void* out, int rank, const int32_t* perm, int elem_bytes, int64_t batch,
+               void* stream) {
+    g_err[0] = 0;
+    if (!in || !out || !perm || rank < 0 || rank > 31 || batch <= 0 || (elem_bytes != 4 && elem_bytes != 8)) {
+        set_err("to_permute: invalid argument");
+        return TO_ERR_INVALID;
+    }
+    // opos[p]: output bit position of input bit position p
+    int opos[32], seen[32] = {0};
+    for (int j = 0; j < rank; ++j) {
+        const int a = perm[j];
+        if (a < 0 || a >= rank || seen[a]) {
+            set_err("to_permute: perm is not a permutation");
+            return TO_ERR_INVALID;
+        }
+        seen[a] = 1;
+        opos[rank - 1 - a] = rank - 1 - j;
+    }
+    int ipos_of_out[32];
+    for (int p = 0; p < rank; ++p) ipos_of_out[opos[p]] = p;
+    // grow m while the tile union stays <= 2^10 entries
+    int m = 0, inU[32] = {0};
+    for (;;) {
+        if (m >= rank) break;
+        int cand[32];
+        memcpy(cand, inU, sizeof(cand));
+        cand[m] = 1;
+        cand[ipos_of_out[m]] = 1;
+        int cnt = 0;
+        for (int p = 0; p < rank; ++p) cnt += cand[p];
+        if (cnt > 10) break;
+        memcpy(inU, cand, sizeof(cand));
+        ++m;
+    }
+    PermPlan p;
+    memset(&p, 0, sizeof(p));
+    p.rank = rank;
+    int nu = 0;
+    for (int q = 0; q < rank; ++q)
+        if (inU[q]) p.u_in[nu++] = (int8_t)q;
+    p.nu = nu;
+    int v = 0;
+    for (int q = 0; q < rank; ++q) {
+        const int ip = ipos_of_out[q];
+        if (inU[ip]) {
+            p.v_out[v] = (int8_t)q;
+            for (int j = 0; j < nu; ++j)
+                if (p.u_in[j] == ip) p.v2u[v] = (int8_t)j;
+            ++v;
+        }
+    }
+    int nr = 0;
+    for (int q = 0; q < rank; ++q) {
+        if (!inU[q]) {
+            p.rest_in[nr] = (int8_t)q;
+            p.rest_out[nr] = (int8_t)opos[q];
+            ++nr;
+        }
+    }
+    p.nrest = nr;
+    const int64_t ntiles = batch << nr;
+    const int grid = (int)(ntiles < 8192 ? ntiles : 8192);
+    const size_t lds = (size_t)elem_bytes << nu;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(k_permute<double>, dim3(grid), dim3(kBlock), lds, st, (const double*)in, (double*)out,
+                           p, batch);
+    else
+        hipLaunchKernelGGL(k_permute<float>, dim3(grid), dim3(kBlock), lds, st, (const float*)in, (float*)out, p,
+                           batch);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TO_OK : hip_fail(e, "to_permute launch");
+}
+
+int to_synchronize(void* stream) {
+    const hipError_t e = hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream));
+    return e == hipSuccess ? TO_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+}  // extern "C"

@@ -1,0 +1,421 @@
+"""The MI355X ``tensor_library``: a drop-in ``BaseTensorAPI`` backed by HIP kernels.
+
+Mirrors the plugin interface the reference's ``execution.run`` drives
+(``tensor_network/tensor_apis/base_api.py:8-28``; registry ``tensor_network/__init__.py:12-16``):
+
+* ``add_argument(key, value)``: ``entry_type`` (``float64`` = plain FP64 executor,
+  ``float32`` = FP32 with per-tensor power-of-two scaling), ``thread_limit`` (accepted;
+  host threads do no arithmetic here), ``max_batch``, ``mem_limit`` (bytes of device
+  memory the executor may use), ``graph`` (capture waves in a HIP graph); any other key
+  raises ``ValueError`` (``base_api.py:9-12``).
+* ``contract_sliced(plan, num_slice_limit)``: all slices of the plan, batched in waves on
+  the device, summed in FP64 (mantissa, exponent) -- the override pattern of
+  ``jax_apis.py:316-322``; returns a Python float.
+* ``contract(network, tree)``: one (possibly sliced) network; returns a 0-d array.
+* ``create_tensor`` / ``reshape`` / ``transpose`` build leaf values on the host (leaves
+  are tiny), ``tensordot`` contracts two tensors on the device (the per-join primitive
+  ``network.identify`` calls, ``tensor_network.pyx:152-154``).
+
+Device memory is allocated through PyTorch; the arithmetic runs only in the HIP library.
+If the library or the GPU is missing, construction raises.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib, schedule
+from .api import BaseTensorAPI, OutOfMemoryError
+from .tree import TreeContext
+
+
+class DeviceTensor:
+    """A dense tensor in device memory (entry type of its API), optionally with a
+    power-of-two scale record (FP32 mode)."""
+
+    def __init__(self, data, shape, rec=None, api=None):
+        self.data = data
+        self.shape = tuple(shape)
+        self.rec = rec
+        self._api = api
+
+    @property
+    def rank(self):
+        return len(self.shape)
+
+    def numpy(self):
+        vals = self.data.double().cpu().numpy().reshape(self.shape)
+        if self.rec is not None:
+            exp = int(self.rec.cpu().numpy().view(np.int64)[0])
+            vals = np.ldexp(vals, exp)
+        return vals
+
+    def __getitem__(self, item):
+        return self.numpy()[item]
+
+
+class Result:
+    """A count as FP64 mantissa and int64 exponent (value = mant * 2^exp)."""
+
+    def __init__(self, mant, exp, slices):
+        self.mant, self.exp, self.slices = float(mant), int(exp), int(slices)
+
+    @property
+    def value(self):
+        try:
+            return math.ldexp(self.mant, self.exp)
+        except OverflowError:
+            return math.copysign(math.inf, self.mant)
+
+    @property
+    def log10(self):
+        if self.mant == 0:
+            return -math.inf
+        return math.log10(abs(self.mant)) + self.exp * math.log10(2.0)
+
+
+class Executable:
+    """A compiled plan resident on the device.  ``run()`` is one full contraction: reset
+    the accumulator, prologue (slice-invariant subtrees), then every wave of slices,
+    replayed from HIP graphs when the API's ``graph`` option is on."""
+
+    def __init__(self, api, prog, total_slices):
+        self.api = api
+        self.prog = prog
+        self.total = total_slices
+        self.nwaves = 0 if prog.root_const else -(-total_slices // prog.nbatch)
+        begin = _lib.Launch(op=_lib.TO_OP_STEP_BEGIN, grid=1, aux2=prog.accum.data_ptr(),
+                            aux3=prog.params.data_ptr())
+        self.step_launches = [begin] + list(prog.prologue)
+        self.wave_launches = list(prog.wave)
+        self._graphs = []
+        self._pro_graph = None
+        self._wave_graph = None
+        if api._use_graph:
+            self._pro_graph = self._capture(self.step_launches)
+            if self.nwaves > 1:
+                self._wave_graph = self._capture(self.wave_launches)
+
+    def _capture(self, launches):
+        arr = _lib.launch_array(launches)
+        graph = ctypes.c_void_p()
+        _lib.check(self.api._lib.to_graph_create(ctypes.addressof(arr), len(launches), self.api._entry_type,
+                                                 ctypes.c_void_p(self.api._stream.cuda_stream),
+                                                 ctypes.byref(graph)), "to_graph_create")
+        self._graphs.append(graph)
+        return graph
+
+    def _launch_graph(self, graph):
+        _lib.check(self.api._lib.to_graph_launch(graph, ctypes.c_void_p(self.api._stream.cuda_stream)),
+                   "to_graph_launch")
+
+    def enqueue(self):
+        """Queue one full contraction on the API's stream (no synchronisation)."""
+        if self._pro_graph is not None:
+            self._launch_graph(self._pro_graph)
+        else:
+            self.api._run(self.step_launches)
+        for _ in range(self.nwaves):
+            if self._wave_graph is not None:
+                self._launch_graph(self._wave_graph)
+            else:
+                self.api._run(self.wave_launches)
+
+    def result(self):
+        """Synchronise and read the (mantissa, exponent) slice sum."""
+        api = self.api
+        api.synchronize()
+        with api._torch.cuda.stream(api._stream):
+            acc_bytes = self.prog.accum[: ctypes.sizeof(_lib.Accum)].cpu().numpy().tobytes()
+        acc = _lib.Accum.from_buffer_copy(acc_bytes)
+        res = Result(acc.mant, acc.exp, acc.count)
+        api.last_result = res
+        api.last_program = self.prog
+        api.last_stats = dict(self.prog.stats, nbatch=self.prog.nbatch, waves=self.nwaves, slices=self.total)
+        return res
+
+    def run(self):
+        self.enqueue()
+        return self.result()
+
+    def close(self):
+        for g in self._graphs:
+            self.api._lib.to_graph_destroy(g)
+        self._graphs = []
+        self._pro_graph = self._wave_graph = None
+
+
+class HipAPI(BaseTensorAPI):
+    def __init__(self, device=0):
+        import torch
+
+        self._torch = torch
+        self._lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError("HipAPI needs a ROCm GPU; none is visible")
+        self._device = torch.device("cuda", device)
+        self._stream = torch.cuda.Stream(device=self._device)
+        self._entry_type = _lib.TO_F64
+        self._thread_limit = None
+        self._max_batch = 1 << 16
+        self._mem_limit = None
+        self._use_graph = True
+        self.last_result = None
+        self.last_program = None
+        self.last_stats = {}
+
+    # -- plugin interface ---------------------------------------------------------
+
+    def add_argument(self, key, value):
+        if key == "entry_type":
+            types = {"float64": _lib.TO_F64, "float32": _lib.TO_F32S}
+            if value not in types:
+                raise ValueError("Unknown hip type %s" % value)
+            self._entry_type = types[value]
+        elif key == "thread_limit":
+            self._thread_limit = value
+        elif key == "max_batch":
+            self._max_batch = max(1, int(value))
+        elif key == "mem_limit":
+            self._mem_limit = None if value is None else int(value)
+        elif key == "graph":
+            self._use_graph = bool(value)
+        else:
+            super().add_argument(key, value)
+
+    def get_entry_size(self):
+        return 8 if self._entry_type == _lib.TO_F64 else 4
+
+    def warm(self):
+        a = self.create_tensor((2,), 2)
+        self.tensordot(a, a, ([0], [0]))
+        self.synchronize()
+
+    def create_tensor(self, shape, default_value=None):
+        if default_value is None:
+            return np.empty(shape, dtype=np.float64)
+        return np.full(shape, default_value, dtype=np.float64)
+
+    def reshape(self, tensor, new_shape):
+        return np.reshape(tensor, new_shape)
+
+    def transpose(self, tensor, perm):
+        return np.transpose(tensor, axes=perm)
+
+    # -- device helpers -----------------------------------------------------------
+
+    @property
+    def _esize(self):
+        return self.get_entry_size()
+
+    def _alloc(self, nbytes):
+        try:
+            return self._torch.empty(max(1, int(nbytes)), dtype=self._torch.uint8, device=self._device)
+        except self._torch.OutOfMemoryError:
+            raise OutOfMemoryError
+
+    def _upload(self, dst, host_bytes):
+        src = self._torch.from_numpy(np.frombuffer(host_bytes, dtype=np.uint8).copy())
+        with self._torch.cuda.stream(self._stream):
+            dst[: src.numel()].copy_(src, non_blocking=False)
+
+    def _run(self, launches):
+        if not launches:
+            return
+        arr = _lib.launch_array(launches)
+        rc = self._lib.to_run(ctypes.addressof(arr), len(launches), self._entry_type,
+                              ctypes.c_void_p(self._stream.cuda_stream))
+        if rc == _lib.TO_ERR_OOM:
+            raise OutOfMemoryError
+        _lib.check(rc, "to_run")
+
+    def synchronize(self):
+        _lib.check(self._lib.to_synchronize(ctypes.c_void_p(self._stream.cuda_stream)), "to_synchronize")
+
+    def _budget(self):
+        free, _total = self._torch.cuda.mem_get_info(self._device)
+        budget = int(free * 0.85)
+        if self._mem_limit is not None:
+            budget = min(budget, self._mem_limit)
+        return budget
+
+    # -- compiled execution ---------------------------------------------------------
+
+    def _compile(self, tree, tables, num_groups, total_slices):
+        """Compile and upload a plan; returns an ``Executable``."""
+        nodes = schedule.analyse(tree, tables)
+        budget = self._budget()
+        fixed = sum(t.rows.size for t in tables.values()) * self._esize
+        W = max(1, min(self._max_batch, total_slices))
+        while True:
+            peak = schedule.peak_bytes(nodes, tables, W, self._entry_type)
+            if peak + fixed <= budget:
+                break
+            if W == 1:
+                raise OutOfMemoryError
+            W = max(1, W // 2)
+        prog = schedule.compile_program(tree, tables, num_groups, self._entry_type, W, self._alloc)
+        self._upload(prog.table, prog.table_host.tobytes())
+        self._upload(prog.coef, prog.coef_host.tobytes())
+        self._upload(prog.blob, prog.blob_host.tobytes())
+        if self._entry_type == _lib.TO_F32S:
+            self._upload(prog.recs, schedule.rec_init_bytes(prog))
+        self._upload(prog.params, np.array([0, total_slices, 0, 1], dtype=np.int64).tobytes())
+        self._upload(prog.accum, bytes(_lib.Accum()))
+        return Executable(self, prog, total_slices)
+
+    def _execute(self, tree, tables, num_groups, total_slices):
+        exe = self._compile(tree, tables, num_groups, total_slices)
+        try:
+            return exe.run()
+        finally:
+            exe.close()
+
+    def prepare(self, execution_plan, num_slice_limit=None):
+        """Compile a sliced plan once; ``Executable.run()`` then contracts all of its
+        slices with inputs already resident on the device."""
+        total, tree, tables, groups = self._plan_inputs(execution_plan, num_slice_limit)
+        return self._compile(tree, tables, groups, total)
+
+    def _plan_inputs(self, execution_plan, num_slice_limit):
+        groups = [g for g in execution_plan.groups_to_slice if len(g) > 0]
+        total, sequences, _assignments = execution_plan.network.get_tensor_slices(
+            execution_plan.groups_to_slice, self.create_tensor
+        )
+        if num_slice_limit is not None:
+            total = min(total, int(num_slice_limit))
+        tree = execution_plan.network.remove_sliced_indices_from(
+            execution_plan.tree, execution_plan.groups_to_slice
+        )
+        tables = {}
+        for t, seq in enumerate(sequences):
+            reordered, lookup, shape = seq.reordered_tensor(self)
+            reordered = np.asarray(reordered, dtype=np.float64)
+            tables[t] = schedule.LeafTable(reordered.reshape(reordered.shape[0], -1), lookup, len(shape))
+        return total, tree, tables, len(groups)
+
+    def contract_sliced(self, execution_plan, num_slice_limit=None):
+        total, tree, tables, groups = self._plan_inputs(execution_plan, num_slice_limit)
+        if total <= 0:
+            return 0
+        return self._execute(tree, tables, groups, total).value
+
+    def contract(self, network, contraction_tree):
+        """Contract one network (sliced tensors keep size-1 axes, which are dropped)."""
+        ctx = TreeContext()
+        stack, tables = [], {}
+        for node in contraction_tree.iterate_postorder():
+            if node.is_leaf:
+                t = node.tensor_index
+                arr = np.asarray(network[t].build(self.create_tensor), dtype=np.float64)
+                keep = [e for e, size in zip(node.free_edges, arr.shape) if size != 1]
+                tables[t] = schedule.LeafTable(arr.reshape(1, -1), [], len(keep))
+                stack.append(ctx.leaf_manual(t, keep))
+            else:
+                right = stack.pop()
+                left = stack.pop()
+                stack.append(ctx.join(left, right))
+        res = self._execute(ctx.get_tree(stack[0]), tables, 0, 1)
+        return np.array(res.value)
+
+    # -- single join ---------------------------------------------------------------
+
+    def _to_device(self, a):
+        if isinstance(a, DeviceTensor):
+            return a
+        arr = np.asarray(a, dtype=np.float64)
+        rec = None
+        if self._entry_type == _lib.TO_F32S:
+            m = float(np.max(np.abs(arr))) if arr.size else 0.0
+            e = math.frexp(m)[1] if m > 0 else 0
+            host = np.ldexp(arr, -e).astype(np.float32)
+            r = _lib.ScaleRec(exp=e, maxbits=int(np.array([1.0], np.float32).view(np.uint32)[0]))
+            rec = self._alloc(ctypes.sizeof(r))
+            self._upload(rec, bytes(r))
+        else:
+            host = arr
+        data = self._alloc(host.nbytes)
+        self._upload(data, np.ascontiguousarray(host).tobytes())
+        view = data[: host.nbytes].view(self._torch.float64 if host.dtype == np.float64 else self._torch.float32)
+        return DeviceTensor(view, arr.shape, rec, self)
+
+    def tensordot(self, a, b, axes):
+        """``numpy.tensordot(a, b, axes)`` on the device; result keeps numpy's axis
+        order (left free axes, then right free axes)."""
+        A, B = self._to_device(a), self._to_device(b)
+        lmap, rmap = list(axes[0]), list(axes[1])
+        for (t, m) in ((A, lmap), (B, rmap)):
+            if any(d != 2 for d in t.shape):
+                raise ValueError("HipAPI tensors must have all dimensions 2")
+        g = schedule.join_geometry(A.rank, B.rank, lmap, rmap)
+        g["var"] = schedule.choose_variant(g, 1, self._esize)
+        out_rank = g["fx"] + g["fy"]
+        out_shape = tuple(A.shape[i] for i in range(A.rank) if i not in lmap) + tuple(
+            B.shape[i] for i in range(B.rank) if i not in rmap)
+        cbuf = self._alloc((1 << out_rank) * self._esize)
+        scaled = self._entry_type == _lib.TO_F32S
+        crec = self._alloc(ctypes.sizeof(_lib.ScaleRec)) if scaled else None
+        X, Y = (A, B) if g["x_is_left"] else (B, A)
+        d = _lib.JoinDesc()
+        d.x, d.y, d.c = X.data.data_ptr(), Y.data.data_ptr(), cbuf.data_ptr()
+        if scaled:
+            d.xs, d.ys, d.cs = X.rec.data_ptr(), Y.rec.data_ptr(), crec.data_ptr()
+        d.fx, d.fy, d.s, d.shf, d.shy = g["fx"], g["fy"], g["s"], g["shf"], g["shy"]
+        d.nbatch = 1
+        d.xf_pos, d.xs_pos = schedule._pos_array(g["xf"]), schedule._pos_array(g["xs"])
+        d.yf_pos, d.ys_pos = schedule._pos_array(g["yf"]), schedule._pos_array(g["ys"])
+        v = g["var"]
+        launches = []
+        if scaled:
+            launches.append(_lib.Launch(op=_lib.TO_OP_SCALE_CLEAR, grid=1, aux0=crec.data_ptr(), iaux0=1))
+        keep = [cbuf, crec]
+        if v["kind"] == "stream":
+            d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
+            blob = self._alloc(ctypes.sizeof(d))
+            self._upload(blob, bytes(d))
+            launches.append(_lib.Launch(op=_lib.TO_OP_JOIN_STREAM, variant=v["yc"], count=1,
+                                        grid=v["nblocks"], lds=v["lds"], descs=blob.data_ptr()))
+            keep.append(blob)
+        else:
+            part = self._alloc(v["nout"] * v["nsplit"] * self._esize)
+            d.partial = part.data_ptr()
+            d.nsplit, d.nblocks = v["nsplit"], v["nblocks"]
+            d2 = _lib.JoinDesc.from_buffer_copy(bytes(d))
+            d2.nblocks = v["fin_blocks"]
+            blob = self._alloc(2 * ctypes.sizeof(d))
+            self._upload(blob, bytes(d) + bytes(d2))
+            launches.append(_lib.Launch(op=_lib.TO_OP_JOIN_SPLITK, count=1, grid=v["nblocks"],
+                                        descs=blob.data_ptr()))
+            launches.append(_lib.Launch(op=_lib.TO_OP_JOIN_FINISH, count=1, grid=v["fin_blocks"],
+                                        descs=blob.data_ptr() + ctypes.sizeof(d)))
+            keep += [blob, part]
+        self._run(launches)
+        self.synchronize()
+        tdtype = self._torch.float64 if not scaled else self._torch.float32
+        view = cbuf[: (1 << out_rank) * self._esize].view(tdtype)
+        return DeviceTensor(view, out_shape, crec, self)
+
+    def permute(self, tensor, perm):
+        """Device ``numpy.transpose`` of an all-dimension-2 tensor (returns numpy)."""
+        A = self._to_device(tensor)
+        out = self._alloc((1 << A.rank) * self._esize)
+        p = (ctypes.c_int32 * max(1, len(perm)))(*perm)
+        rc = self._lib.to_permute(ctypes.c_void_p(A.data.data_ptr()), ctypes.c_void_p(out.data_ptr()), A.rank,
+                                  ctypes.addressof(p), self._esize, 1, ctypes.c_void_p(self._stream.cuda_stream))
+        _lib.check(rc, "to_permute")
+        self.synchronize()
+        tdtype = self._torch.float64 if self._esize == 8 else self._torch.float32
+        view = out[: (1 << A.rank) * self._esize].view(tdtype)
+        return DeviceTensor(view, tuple(A.shape[i] for i in perm), A.rec, self).numpy()
+
+
+class HipF32API(HipAPI):
+    """The FP32-scaled executor as its own library choice."""
+
+    def __init__(self, device=0):
+        super().__init__(device)
+        self._entry_type = _lib.TO_F32S
+
+
+HIP_APIS = {"hip": HipAPI, "hip-f32": HipF32API}

@@ -1,0 +1,483 @@
+"""Schedule compiler: a contraction tree plus per-leaf slice tables -> launch records.
+
+The reference walks the tree once per slice, calling tensordot on every join
+(``tensor_network.pyx:142-156``).  Here the walk is compiled once:
+
+* every join becomes one index-bit-gather kernel launch (``to_join_desc``), with the
+  operand that has more free bits streamed and the other staged in LDS; numpy's output
+  order (left free axes, then right free axes; ``contraction_tree.pyx:264-282``) is
+  produced directly, so no operand is ever permuted;
+* slices run in batches ("waves") of W: a tensor depending on a sliced leaf carries a
+  batch dimension, every other tensor is computed once in a prologue and broadcast
+  (batch stride 0) -- slice-invariant subtrees are not recomputed per slice;
+* leaves follow the batched-slice design of ``jax_apis.py:168-251``: each leaf is
+  reordered once to ``[slice row, unsliced axes]`` and a wave gathers row
+  ``sum_g lookup[g] * assignment[g]`` for each of its slices on the device;
+* buffers are placed by simulating the post-order walk with a first-fit allocator, so
+  the arena size is the true peak for a given W, and W is the largest batch that fits.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import JoinDesc, LeafDesc, Launch
+
+ALIGN = 256
+STREAM_MAX_S = 12
+LDS_LIMIT = 64 * 1024
+MAX_STREAM_BLOCKS = 4096
+
+
+class _Arena:
+    """First-fit placement of buffers in one linear address space (bytes)."""
+
+    def __init__(self):
+        self.free = [(0, 1 << 62)]
+        self.peak = 0
+
+    def alloc(self, nbytes):
+        nbytes = max(ALIGN, (nbytes + ALIGN - 1) // ALIGN * ALIGN)
+        for i, (start, size) in enumerate(self.free):
+            if size >= nbytes:
+                if size == nbytes:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (start + nbytes, size - nbytes)
+                self.peak = max(self.peak, start + nbytes)
+                return start, nbytes
+        raise MemoryError("arena exhausted")
+
+    def release(self, block):
+        start, size = block
+        self.free.append((start, size))
+        self.free.sort()
+        merged = []
+        for s, z in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == s:
+                merged[-1] = (merged[-1][0], merged[-1][1] + z)
+            else:
+                merged.append((s, z))
+        self.free = merged
+
+
+class Node:
+    __slots__ = ("idx", "is_leaf", "tensor", "free", "left", "right", "lmap", "rmap",
+                 "rank", "batched", "parent", "geom", "buf", "ptr", "bstride", "rec", "rec_stride")
+
+    def __init__(self, idx):
+        self.idx = idx
+        self.parent = -1
+        self.buf = None
+        self.rec = None
+        self.rec_stride = 0
+
+
+def flatten_tree(tree):
+    """Post-order node list (children before parents) of a reference-style tree."""
+    nodes, stack = [], []
+    for t in tree.iterate_postorder():
+        n = Node(len(nodes))
+        n.is_leaf = bool(t.is_leaf)
+        n.free = list(t.free_edges)
+        n.rank = len(n.free)
+        if n.is_leaf:
+            n.tensor = t.tensor_index
+        else:
+            n.right = stack.pop()
+            n.left = stack.pop()
+            n.lmap = list(t.left_edge_map)
+            n.rmap = list(t.right_edge_map)
+            nodes[n.left].parent = n.idx
+            nodes[n.right].parent = n.idx
+        nodes.append(n)
+        stack.append(n.idx)
+    if len(stack) != 1:
+        raise ValueError("tree walk did not end in one root")
+    return nodes
+
+
+def join_geometry(lrank, rrank, lmap, rmap):
+    """Bit placement of one join in numpy tensordot order.
+
+    Output index = (A free index << b) | (B free index), with A = left, B = right, the
+    free indices C-ordered over the unshared axes.  Axis i of a rank-r tensor is bit
+    r-1-i.  Returns the X (streamed) / Y roles and bit positions."""
+    s = len(lmap)
+    fa = [i for i in range(lrank) if i not in set(lmap)]
+    fb = [i for i in range(rrank) if i not in set(rmap)]
+    a, b = len(fa), len(fb)
+    a_free = [lrank - 1 - fa[a - 1 - t] for t in range(a)]
+    b_free = [rrank - 1 - fb[b - 1 - t] for t in range(b)]
+    a_sh = [lrank - 1 - lmap[s - 1 - t] for t in range(s)]
+    b_sh = [rrank - 1 - rmap[s - 1 - t] for t in range(s)]
+    x_is_left = (a, lrank) >= (b, rrank)
+    if x_is_left:
+        return dict(x_is_left=True, fx=a, fy=b, s=s, shf=b, shy=0,
+                    xf=a_free, xs=a_sh, yf=b_free, ys=b_sh)
+    return dict(x_is_left=False, fx=b, fy=a, s=s, shf=0, shy=b,
+                xf=b_free, xs=b_sh, yf=a_free, ys=a_sh)
+
+
+def choose_variant(g, nb, esize):
+    """Kernel variant and launch shape for a join with ``nb`` batch items."""
+    fx, fy, s = g["fx"], g["fy"], g["s"]
+    K = 1 << s
+    yc = min(16, 1 << fy)
+    if fx >= 8:
+        while yc > 1 and 8 * K + yc * K * esize > LDS_LIMIT:
+            yc //= 2
+        lds = 8 * K + yc * K * esize
+    else:
+        lds = 8 * K
+    nych = (1 << fy) // yc
+    threads = (nb << fx) * nych
+    stream_ok = s <= STREAM_MAX_S and lds <= LDS_LIMIT
+    if stream_ok and not (s >= 8 and threads < 32768):
+        if fx >= 8:
+            ntiles = nb * (1 << (fx - 8)) * nych
+        else:
+            ntiles = -(-(nb << fx) // 256) * nych
+        return dict(kind="stream", yc=yc, nych=nych, lds=lds,
+                    nblocks=int(min(ntiles, MAX_STREAM_BLOCKS)))
+    nout = nb << (fx + fy)
+    nsplit = 1
+    if s >= 8:
+        nsplit = 1 << int(min(s - 8, max(0, math.floor(math.log2(max(1, 4096 // nout))))))
+    return dict(kind="splitk", nsplit=nsplit, nout=nout, nblocks=int(nout * nsplit),
+                fin_blocks=int(min(-(-nout // 256), MAX_STREAM_BLOCKS)))
+
+
+class LeafTable:
+    """Per-tensor slice rows: ``rows`` (float64 [nrows, 2^rank]) and ``coef`` (one
+    integer per slice group: row = sum_g coef[g] * assignment[g])."""
+
+    __slots__ = ("rows", "coef", "rank")
+
+    def __init__(self, rows, coef, rank):
+        self.rows = np.ascontiguousarray(rows, dtype=np.float64).reshape(-1, 1 << rank)
+        self.coef = [int(c) for c in coef]
+        self.rank = rank
+
+    @property
+    def batched(self):
+        return any(self.coef)
+
+
+class Program:
+    """A compiled plan: memory layout, descriptor blob and launch records."""
+
+    def __init__(self):
+        self.prologue = []
+        self.wave = []
+        self.arena_bytes = 0
+        self.nbatch = 1
+        self.root_const = False
+        self.stats = {}
+
+
+def _pos_array(vals):
+    arr = _lib.c_i8x32()
+    for i, v in enumerate(vals):
+        arr[i] = v
+    return arr
+
+
+def plan_memory(nodes, tables, W, esize):
+    """Arena placement for batch W.  Returns (arena peak bytes, partial placements)."""
+    arena = _Arena()
+    partial = {}
+    # prologue: constant joins in post-order
+    for n in nodes:
+        if n.is_leaf or n.batched:
+            continue
+        n.buf = arena.alloc((1 << n.rank) * esize)
+        if n.geom["var"]["kind"] == "splitk":
+            partial[n.idx] = arena.alloc(n.geom["var"]["nout"] * n.geom["var"]["nsplit"] * esize)
+            arena.release(partial[n.idx])
+        for c in (n.left, n.right):
+            ch = nodes[c]
+            if not ch.is_leaf:
+                arena.release(ch.buf)
+    # wave: batched leaves are gathered together at wave start
+    for n in nodes:
+        if n.is_leaf and n.batched:
+            n.buf = arena.alloc(W * (1 << n.rank) * esize)
+    for n in nodes:
+        if n.is_leaf or not n.batched:
+            continue
+        n.buf = arena.alloc(W * (1 << n.rank) * esize)
+        if n.geom["var"]["kind"] == "splitk":
+            partial[n.idx] = arena.alloc(n.geom["var"]["nout"] * n.geom["var"]["nsplit"] * esize)
+            arena.release(partial[n.idx])
+        for c in (n.left, n.right):
+            ch = nodes[c]
+            if ch.batched:
+                arena.release(ch.buf)
+    return arena.peak, partial
+
+
+def analyse(tree, tables):
+    """Node list with batched flags (depends on a sliced leaf)."""
+    nodes = flatten_tree(tree)
+    for n in nodes:
+        if n.is_leaf:
+            t = tables[n.tensor]
+            if t.rank != n.rank:
+                raise ValueError("leaf %d: table rank %d != tree rank %d" % (n.tensor, t.rank, n.rank))
+            n.batched = t.batched
+        else:
+            n.batched = nodes[n.left].batched or nodes[n.right].batched
+    if nodes[-1].rank != 0:
+        raise ValueError("the tree's root has free edges; only full contractions are supported")
+    return nodes
+
+
+def peak_bytes(nodes, tables, W, entry_type):
+    esize = 8 if entry_type == _lib.TO_F64 else 4
+    for n in nodes:
+        if not n.is_leaf:
+            nb = W if n.batched else 1
+            g = join_geometry(nodes[n.left].rank, nodes[n.right].rank, n.lmap, n.rmap)
+            g["var"] = choose_variant(g, nb, esize)
+            n.geom = g
+    peak, _ = plan_memory(nodes, tables, W, esize)
+    return peak
+
+
+def compile_program(tree, tables, num_groups, entry_type, W, device_alloc):
+    """Build a Program for batch size W.
+
+    ``device_alloc(nbytes)`` returns an object with ``.data_ptr()`` (device memory that
+    stays alive as long as the Program).  ``tables`` maps tensor id -> LeafTable."""
+    esize = 8 if entry_type == _lib.TO_F64 else 4
+    scaled = entry_type == _lib.TO_F32S
+    fdtype = np.float64 if esize == 8 else np.float32
+    nodes = analyse(tree, tables)
+    prog = Program()
+    prog.nbatch = W
+    prog.nodes = nodes
+
+    for n in nodes:
+        if not n.is_leaf:
+            nb = W if n.batched else 1
+            g = join_geometry(nodes[n.left].rank, nodes[n.right].rank, n.lmap, n.rmap)
+            g["var"] = choose_variant(g, nb, esize)
+            n.geom = g
+    peak, partial = plan_memory(nodes, tables, W, esize)
+    prog.arena_bytes = peak
+
+    # ---- leaf tables (host) -> one device buffer
+    leaf_ids = sorted({n.tensor for n in nodes if n.is_leaf})
+    table_off, chunks, exp0 = {}, [], {}
+    off = 0
+    for t in leaf_ids:
+        rows = tables[t].rows
+        if scaled:
+            m = float(np.max(np.abs(rows))) if rows.size else 0.0
+            e = math.frexp(m)[1] if m > 0 else 0
+            exp0[t] = e
+            rows = np.ldexp(rows, -e)
+        arr = np.ascontiguousarray(rows, dtype=fdtype).ravel()
+        table_off[t] = off
+        chunks.append(arr)
+        off += (arr.size * esize + ALIGN - 1) // ALIGN * ALIGN // esize
+    table_host = np.zeros(max(off, 1), dtype=fdtype)
+    for t, arr in zip(leaf_ids, chunks):
+        table_host[table_off[t]:table_off[t] + arr.size] = arr
+    coef_host = np.zeros(max(1, len(leaf_ids) * max(1, num_groups)), dtype=np.int64)
+    coef_off = {}
+    for i, t in enumerate(leaf_ids):
+        c = tables[t].coef
+        coef_off[t] = i * max(1, num_groups)
+        coef_host[coef_off[t]:coef_off[t] + len(c)] = c
+
+    # ---- scale records: [constant leaves][constant joins][wave tensors]
+    nrec = 0
+    rec_init = []
+    if scaled:
+        for n in nodes:
+            if n.is_leaf and not n.batched:
+                n.rec, n.rec_stride = nrec, 0
+                rec_init.append((nrec, exp0[n.tensor]))
+                nrec += 1
+        pro_rec0 = nrec
+        for n in nodes:
+            if not n.is_leaf and not n.batched:
+                n.rec, n.rec_stride = nrec, 0
+                nrec += 1
+        wave_rec0 = nrec
+        for n in nodes:
+            if n.batched:
+                n.rec, n.rec_stride = nrec, 1
+                nrec += W
+        prog.rec_ranges = (pro_rec0, wave_rec0, nrec)
+
+    # ---- device buffers
+    prog.table = device_alloc(table_host.nbytes)
+    prog.coef = device_alloc(coef_host.nbytes)
+    prog.arena = device_alloc(max(ALIGN, peak))
+    prog.recs = device_alloc(max(1, nrec) * ctypes.sizeof(_lib.ScaleRec))
+    prog.params = device_alloc(4 * 8)  # wave params [begin, end], prologue params [0, 1]
+    prog.accum = device_alloc(ctypes.sizeof(_lib.Accum))
+    prog.table_host, prog.coef_host = table_host, coef_host
+    tbase, abase, rbase = prog.table.data_ptr(), prog.arena.data_ptr(), prog.recs.data_ptr()
+    rec_size = ctypes.sizeof(_lib.ScaleRec)
+    prog.rec_init = rec_init
+    wave_params = prog.params.data_ptr()
+    pro_params = wave_params + 16
+
+    for n in nodes:
+        if n.is_leaf and not n.batched:
+            n.ptr = tbase + table_off[n.tensor] * esize
+            n.bstride = 0
+        else:
+            n.ptr = abase + n.buf[0]
+            n.bstride = (1 << n.rank) if n.batched else 0
+
+    def recp(n):
+        return rbase + n.rec * rec_size if scaled else 0
+
+    # ---- descriptors
+    blob = bytearray()
+
+    def put(structs):
+        nonlocal blob
+        start = (len(blob) + ALIGN - 1) // ALIGN * ALIGN
+        blob += b"\0" * (start - len(blob))
+        for st in structs:
+            blob += bytes(st)
+        return start
+
+    pending = []  # (list, Launch, blob offset attr)
+
+    def join_launches(n, phase_list):
+        g, v = n.geom, n.geom["var"]
+        nb = W if n.batched else 1
+        X, Y = (nodes[n.left], nodes[n.right]) if g["x_is_left"] else (nodes[n.right], nodes[n.left])
+        d = JoinDesc()
+        d.x, d.y, d.c = X.ptr, Y.ptr, n.ptr
+        d.xb = X.bstride if n.batched else 0
+        d.yb = Y.bstride if n.batched else 0
+        d.cb = n.bstride
+        if scaled:
+            d.xs, d.ys, d.cs = recp(X), recp(Y), recp(n)
+            d.xsb = X.rec_stride if n.batched else 0
+            d.ysb = Y.rec_stride if n.batched else 0
+        d.fx, d.fy, d.s = g["fx"], g["fy"], g["s"]
+        d.shf, d.shy = g["shf"], g["shy"]
+        d.nbatch = nb
+        d.block0 = 0
+        d.xf_pos, d.xs_pos = _pos_array(g["xf"]), _pos_array(g["xs"])
+        d.yf_pos, d.ys_pos = _pos_array(g["yf"]), _pos_array(g["ys"])
+        if v["kind"] == "stream":
+            d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
+            off = put([d])
+            phase_list.append(("stream", Launch(op=_lib.TO_OP_JOIN_STREAM, variant=v["yc"], count=1,
+                                                grid=v["nblocks"], lds=v["lds"]), off, n.idx))
+        else:
+            d.partial = abase + partial[n.idx][0]
+            d.nsplit = v["nsplit"]
+            d.nblocks = v["nblocks"]
+            off = put([d])
+            phase_list.append(("splitk", Launch(op=_lib.TO_OP_JOIN_SPLITK, count=1, grid=v["nblocks"]), off, n.idx))
+            d2 = JoinDesc.from_buffer_copy(bytes(d))
+            d2.nblocks = v["fin_blocks"]
+            off2 = put([d2])
+            phase_list.append(("finish", Launch(op=_lib.TO_OP_JOIN_FINISH, count=1, grid=v["fin_blocks"]), off2, n.idx))
+
+    pro, wave = [], []
+    if scaled and prog.rec_ranges[1] > prog.rec_ranges[0]:
+        a, b = prog.rec_ranges[0], prog.rec_ranges[1]
+        pro.append(("clear", Launch(op=_lib.TO_OP_SCALE_CLEAR, grid=max(1, min(1024, -(-(b - a) // 256))),
+                                    aux0=rbase + a * rec_size, iaux0=b - a), None, None))
+    for n in nodes:
+        if not n.is_leaf and not n.batched:
+            join_launches(n, pro)
+    root = nodes[-1]
+    prog.root_const = not root.batched
+    if scaled and prog.rec_ranges[2] > prog.rec_ranges[1]:
+        a, b = prog.rec_ranges[1], prog.rec_ranges[2]
+        wave.append(("clear", Launch(op=_lib.TO_OP_SCALE_CLEAR, grid=max(1, min(1024, -(-(b - a) // 256))),
+                                     aux0=rbase + a * rec_size, iaux0=b - a), None, None))
+    leaf_descs = []
+    block0 = 0
+    for n in nodes:
+        if n.is_leaf and n.batched:
+            ld = LeafDesc()
+            ld.src = tbase + table_off[n.tensor] * esize
+            ld.dst = n.ptr
+            ld.coef = prog.coef.data_ptr() + coef_off[n.tensor] * 8
+            ld.rec = recp(n)
+            ld.exp0 = exp0.get(n.tensor, 0)
+            ld.rank = n.rank
+            ld.ngroups = num_groups
+            ld.nbatch = W
+            ld.block0 = block0
+            ld.nblocks = int(min(1024, -(-(W << n.rank) // 256)))
+            block0 += ld.nblocks
+            leaf_descs.append(ld)
+    if leaf_descs:
+        off = put(leaf_descs)
+        wave.append(("leaf", Launch(op=_lib.TO_OP_LEAF_GATHER, count=len(leaf_descs), grid=block0,
+                                    aux3=wave_params), off, None))
+    for n in nodes:
+        if not n.is_leaf and n.batched:
+            join_launches(n, wave)
+    root_launch = Launch(op=_lib.TO_OP_ROOT_REDUCE, grid=1, aux0=root.ptr,
+                         aux1=recp(root) if scaled else 0, aux2=prog.accum.data_ptr(),
+                         aux3=pro_params if prog.root_const else wave_params,
+                         iaux0=1 if prog.root_const else W)
+    if prog.root_const:
+        pro.append(("root", root_launch, None, None))
+    else:
+        wave.append(("root", root_launch, None, None))
+        wave.append(("advance", Launch(op=_lib.TO_OP_WAVE_ADVANCE, grid=1, aux3=wave_params, iaux0=W), None, None))
+
+    prog.blob_host = np.frombuffer(bytes(blob) or b"\0", dtype=np.uint8).copy()
+    prog.blob = device_alloc(prog.blob_host.nbytes)
+    bbase = prog.blob.data_ptr()
+    prog.prologue_info, prog.wave_info = [], []
+    for lst, out, info in ((pro, prog.prologue, prog.prologue_info), (wave, prog.wave, prog.wave_info)):
+        for kind, L, off, nidx in lst:
+            if off is not None:
+                L.descs = bbase + off
+            out.append(L)
+            info.append((kind, nidx))
+    prog.stats = dict(
+        nodes=len(nodes),
+        joins=sum(1 for n in nodes if not n.is_leaf),
+        batched_joins=sum(1 for n in nodes if not n.is_leaf and n.batched),
+        splitk=sum(1 for n in nodes if not n.is_leaf and n.geom["var"]["kind"] == "splitk"),
+        arena_bytes=peak,
+        table_bytes=int(table_host.nbytes),
+        launches_prologue=len(prog.prologue),
+        launches_wave=len(prog.wave),
+    )
+    return prog
+
+
+def join_traffic(node, nodes, W, esize):
+    """Algorithmic (bytes, flops) of one join: each distinct operand entry read once,
+    each output entry written once; flops = 2 * outputs * 2^s."""
+    nb = W if node.batched else 1
+    g = node.geom
+    total = 0
+    for ch in (nodes[node.left], nodes[node.right]):
+        total += (1 << ch.rank) * (nb if ch.batched else 1)
+    total += (1 << node.rank) * nb
+    flops = 2 * nb * (1 << (g["fx"] + g["fy"] + g["s"]))
+    return total * esize, flops
+
+
+def rec_init_bytes(prog):
+    """Initial scale records (constant leaves: exponent exp0, max |entry| 1.0)."""
+    n = max(1, prog.rec_ranges[2] if hasattr(prog, "rec_ranges") else 1)
+    recs = (_lib.ScaleRec * n)()
+    one = np.array([1.0], dtype=np.float32).view(np.uint32)[0]
+    for i, e in getattr(prog, "rec_init", []):
+        recs[i].exp = e
+        recs[i].maxbits = int(one)
+    return bytes(recs)

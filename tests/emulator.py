@@ -1,0 +1,233 @@
+"""Test-only numpy emulator of the launch records in ``include/tensororder_hip.h``.
+
+It executes a compiled Program's records against a simulated device heap so the schedule
+compiler (bit placement, memory plan, batching, leaf gather, root reduction) can be
+checked on a machine without a GPU.  It restates the kernels' documented semantics, not
+their code, and is never used by the product.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+
+from tensororder_amd import _lib
+
+HEAP_BASE = 1 << 40
+
+
+class Heap:
+    """A flat byte heap handing out fake device addresses."""
+
+    def __init__(self, nbytes=1 << 28):
+        self.mem = np.zeros(nbytes, dtype=np.uint8)
+        self.top = 0
+
+    def alloc(self, nbytes):
+        start = (self.top + 255) // 256 * 256
+        self.top = start + int(nbytes)
+        if self.top > self.mem.size:
+            raise MemoryError("emulated heap exhausted")
+        heap = self
+
+        class Buf:
+            def data_ptr(self_inner):
+                return HEAP_BASE + start
+
+            def __getitem__(self_inner, sl):
+                return heap.mem[start:][sl]
+
+        return Buf()
+
+    def view(self, addr, dtype, count):
+        off = addr - HEAP_BASE
+        nbytes = np.dtype(dtype).itemsize * count
+        return self.mem[off:off + nbytes].view(dtype)
+
+    def write(self, buf, data):
+        off = buf.data_ptr() - HEAP_BASE
+        self.mem[off:off + len(data)] = np.frombuffer(data, dtype=np.uint8)
+
+    def read_struct(self, cls, addr):
+        off = addr - HEAP_BASE
+        return cls.from_buffer_copy(self.mem[off:off + ctypes.sizeof(cls)].tobytes())
+
+
+def pdep_array(vals, pos, n):
+    out = np.zeros(len(vals), dtype=np.int64)
+    for i in range(n):
+        out |= ((vals >> i) & 1) << int(pos[i])
+    return out
+
+
+def _scale_exp(bits):
+    m = float(np.array([bits], dtype=np.uint32).view(np.float32)[0])
+    return math.frexp(m)[1] if m > 0 else 0
+
+
+class Emulator:
+    def __init__(self, heap, entry_type):
+        self.heap = heap
+        self.scaled = entry_type == _lib.TO_F32S
+        self.dt = np.float32 if self.scaled else np.float64
+
+    def recs(self, addr, count):
+        raw = self.heap.view(addr, np.uint8, 16 * count).reshape(count, 16)
+        return raw
+
+    def get_rec(self, addr, idx):
+        r = self.heap.read_struct(_lib.ScaleRec, addr + 16 * idx)
+        return r.exp, r.maxbits
+
+    def set_rec(self, addr, idx, exp=None, maxbits=None):
+        off = addr - HEAP_BASE + 16 * idx
+        if exp is not None:
+            self.heap.mem[off:off + 8] = np.frombuffer(np.int64(exp).tobytes(), np.uint8)
+        if maxbits is not None:
+            cur = int(self.heap.mem[off + 8:off + 12].view(np.uint32)[0])
+            new = max(cur, int(maxbits))
+            self.heap.mem[off + 8:off + 12] = np.frombuffer(np.uint32(new).tobytes(), np.uint8)
+
+    def run(self, launches):
+        for L in launches:
+            getattr(self, "op_%d" % L.op)(L)
+
+    # leaf gather
+    def op_1(self, L):
+        for i in range(L.count):
+            d = self.heap.read_struct(_lib.LeafDesc, L.descs + i * ctypes.sizeof(_lib.LeafDesc))
+            begin = int(self.heap.view(L.aux3, np.int64, 1)[0]) if L.aux3 else 0
+            E = 1 << d.rank
+            coef = self.heap.view(d.coef, np.int64, max(1, d.ngroups))
+            for b in range(d.nbatch):
+                sidx = begin + b
+                row = sum(int(coef[g]) * ((sidx >> (d.ngroups - 1 - g)) & 1) for g in range(d.ngroups))
+                src = self.heap.view(d.src + row * E * self.dt().itemsize, self.dt, E)
+                dst = self.heap.view(d.dst + b * E * self.dt().itemsize, self.dt, E)
+                dst[:] = src
+                if self.scaled and d.rec:
+                    one = int(np.array([1.0], np.float32).view(np.uint32)[0])
+                    self.set_rec(d.rec, b, exp=d.exp0)
+                    off = d.rec - HEAP_BASE + 16 * b + 8
+                    self.heap.mem[off:off + 4] = np.frombuffer(np.uint32(one).tobytes(), np.uint8)
+
+    def _join(self, d, write_partial=False, finish=False):
+        dt = self.dt
+        fx, fy, s = d.fx, d.fy, d.s
+        f = np.arange(1 << fx, dtype=np.int64)
+        y = np.arange(1 << fy, dtype=np.int64)
+        k = np.arange(1 << s, dtype=np.int64)
+        xf, xs = pdep_array(f, d.xf_pos, fx), pdep_array(k, d.xs_pos, s)
+        yf, ys = pdep_array(y, d.yf_pos, fy), pdep_array(k, d.ys_pos, s)
+        cidx = (f[:, None] << d.shf) | (y[None, :] << d.shy)
+        esz = np.dtype(dt).itemsize
+        for b in range(d.nbatch):
+            X = self.heap.view(d.x + b * d.xb * esz, dt, int((xf.max(initial=0) + xs.max(initial=0)) + 1))
+            Y = self.heap.view(d.y + b * d.yb * esz, dt, int((yf.max(initial=0) + ys.max(initial=0)) + 1))
+            sx = sy = 1.0
+            ex = ey = 0
+            if self.scaled:
+                e0, m0 = self.get_rec(d.xs, b * d.xsb)
+                e1, m1 = self.get_rec(d.ys, b * d.ysb)
+                sx, ex = 2.0 ** -_scale_exp(m0), e0 + _scale_exp(m0)
+                sy, ey = 2.0 ** -_scale_exp(m1), e1 + _scale_exp(m1)
+            Xm = X[xf[:, None] + xs[None, :]].astype(np.float64) * sx
+            Ym = Y[yf[:, None] + ys[None, :]].astype(np.float64) * sy
+            Cm = (Xm @ Ym.T).astype(dt)
+            C = self.heap.view(d.c + b * d.cb * esz, dt, int(cidx.max() + 1))
+            C[cidx] = Cm
+            if self.scaled:
+                m = float(np.max(np.abs(Cm))) if Cm.size else 0.0
+                self.set_rec(d.cs, b, exp=ex + ey,
+                             maxbits=int(np.array([m], np.float32).view(np.uint32)[0]))
+
+    def op_2(self, L):
+        for i in range(L.count):
+            self._join(self.heap.read_struct(_lib.JoinDesc, L.descs + i * ctypes.sizeof(_lib.JoinDesc)))
+
+    def op_3(self, L):  # split-K partials: the emulator computes the join at FINISH
+        pass
+
+    def op_4(self, L):
+        for i in range(L.count):
+            self._join(self.heap.read_struct(_lib.JoinDesc, L.descs + i * ctypes.sizeof(_lib.JoinDesc)))
+
+    def op_5(self, L):
+        raw = self.recs(L.aux0, L.iaux0)
+        raw[:] = 0
+
+    def op_6(self, L):
+        params = self.heap.view(L.aux3, np.int64, 2)
+        begin, end = int(params[0]), int(params[1])
+        nvalid = max(0, min(int(L.iaux0), end - begin))
+        vals = self.heap.view(L.aux0, self.dt, max(1, int(L.iaux0)))
+        acc = self.heap.read_struct(_lib.Accum, L.aux2)
+        total = acc.mant * 2.0 ** acc.exp if acc.mant else 0.0
+        for b in range(nvalid):
+            v = float(vals[b])
+            if self.scaled:
+                v = v * 2.0 ** self.get_rec(L.aux1, b)[0]
+            total += v
+        m, e = math.frexp(total)
+        acc.mant, acc.exp, acc.count = m, (e if m else 0), acc.count + nvalid
+        off = L.aux2 - HEAP_BASE
+        self.heap.mem[off:off + ctypes.sizeof(_lib.Accum)] = np.frombuffer(bytes(acc), np.uint8)
+
+    def op_7(self, L):
+        params = self.heap.view(L.aux3, np.int64, 2)
+        params[0] += L.iaux0
+
+
+def emulate_plan(tree, tables, num_groups, total, entry_type, W):
+    """Compile and emulate; returns (value, program)."""
+    from tensororder_amd import schedule
+
+    heap = Heap()
+    prog = schedule.compile_program(tree, tables, num_groups, entry_type, W, heap.alloc)
+    heap.write(prog.table, prog.table_host.tobytes())
+    heap.write(prog.coef, prog.coef_host.tobytes())
+    heap.write(prog.blob, prog.blob_host.tobytes())
+    if entry_type == _lib.TO_F32S:
+        heap.write(prog.recs, schedule.rec_init_bytes(prog))
+    heap.write(prog.params, np.array([0, total, 0, 1], dtype=np.int64).tobytes())
+    heap.write(prog.accum, bytes(_lib.Accum()))
+    emu = Emulator(heap, entry_type)
+    emu.run(prog.prologue)
+    if not prog.root_const:
+        for _ in range(-(-total // W)):
+            emu.run(prog.wave)
+    acc = heap.read_struct(_lib.Accum, prog.accum.data_ptr())
+    return math.ldexp(acc.mant, acc.exp), prog
+
+
+def plan_tables(plan, api_like):
+    """(tree, tables, num_groups, total) exactly as HipAPI.contract_sliced builds them."""
+    from tensororder_amd import schedule
+
+    groups = [g for g in plan.groups_to_slice if len(g) > 0]
+    total, sequences, _ = plan.network.get_tensor_slices(plan.groups_to_slice, api_like.create_tensor)
+    tree = plan.network.remove_sliced_indices_from(plan.tree, plan.groups_to_slice)
+    tables = {}
+    for t, seq in enumerate(sequences):
+        reordered, lookup, shape = seq.reordered_tensor(api_like)
+        reordered = np.asarray(reordered, dtype=np.float64)
+        tables[t] = schedule.LeafTable(reordered.reshape(reordered.shape[0], -1), lookup, len(shape))
+    return tree, tables, len(groups), total
+
+
+class HostOps:
+    """The host-side leaf helpers of HipAPI (numpy), without needing a GPU."""
+
+    @staticmethod
+    def create_tensor(shape, default_value=None):
+        if default_value is None:
+            return np.empty(shape, dtype=np.float64)
+        return np.full(shape, default_value, dtype=np.float64)
+
+    @staticmethod
+    def reshape(t, shape):
+        return np.reshape(t, shape)
+
+    @staticmethod
+    def transpose(t, perm):
+        return np.transpose(t, axes=perm)

@@ -1,0 +1,134 @@
+"""The schedule compiler's launch records, executed by the test-only numpy emulator,
+reproduce the oracle: bit placement, batching, slice-row gather, memory plan, split-K and
+the (mantissa, exponent) slice sum, in both entry types."""
+
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+import emulator
+from oracle import numpy_oracle
+from tensororder_amd import _lib, plan_io, schedule
+from tensororder_amd.network import BuiltTensor, TensorNetwork
+from tensororder_amd.plan import SlicedExecutionPlan
+from tensororder_amd.tree import TreeContext
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SMALL = [f for f in sorted(glob.glob(os.path.join(HERE, "golden", "*.json")))
+         if json.load(open(f))["meta"]["maxrank"] <= 16]
+
+
+@pytest.mark.parametrize("path", SMALL, ids=[os.path.basename(p)[:-5] for p in SMALL])
+@pytest.mark.parametrize("et,tol", [(_lib.TO_F64, 1e-12), (_lib.TO_F32S, 1e-6)])
+def test_fixture_emulated(path, et, tol):
+    doc = json.load(open(path))
+    plan = plan_io.plan_from_dict(doc)
+    want = numpy_oracle.contract_sliced(doc)
+    tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
+    for W in sorted({1, 3, total}):
+        got, prog = emulator.emulate_plan(tree, tables, G, total, et, W)
+        assert got == pytest.approx(want, rel=tol)
+
+
+def _two_tensor_plan(ra, rb, shared, seed):
+    """A network of two dense tensors sharing `shared` edges (order scrambled)."""
+    rng = np.random.default_rng(seed)
+    net = TensorNetwork()
+    a = net.add_node(BuiltTensor(rng.standard_normal((2,) * ra)))
+    b = net.add_node(BuiltTensor(rng.standard_normal((2,) * rb)))
+    pa = rng.permutation(ra)[:shared]
+    pb = rng.permutation(rb)[:shared]
+    for i, j in zip(pa, pb):
+        net.connect(a[i][0], a[i][1], b[j][0], b[j][1])
+    ctx = TreeContext()
+    # remaining free axes get connected to rank-1 dense tensors so the root is a scalar
+    leaves = []
+    for t, r in ((0, ra), (1, rb)):
+        for ax in range(r):
+            if net._index_lists[t][ax] < 0:
+                u = net.add_node(BuiltTensor(rng.standard_normal((2,))))
+                net.connect(t, ax, u[0][0], 0)
+                leaves.append(u[0][0])
+    root = ctx.join(ctx.leaf(net, 0), ctx.leaf(net, 1))
+    for u in leaves:
+        root = ctx.join(root, ctx.leaf(net, u))
+    return SlicedExecutionPlan(ctx.get_tree(root), net)
+
+
+@pytest.mark.parametrize("ra,rb,s", [(12, 12, 12), (13, 11, 10), (9, 14, 9), (6, 6, 0), (10, 3, 1)])
+@pytest.mark.parametrize("et,tol", [(_lib.TO_F64, 1e-11), (_lib.TO_F32S, 2e-5)])
+def test_dense_joins_emulated(ra, rb, s, et, tol):
+    plan = _two_tensor_plan(ra, rb, s, seed=ra * 100 + rb)
+    doc = plan_io.plan_to_dict(plan)
+    want = numpy_oracle.contract_sliced(doc)
+    tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
+    got, prog = emulator.emulate_plan(tree, tables, G, total, et, 1)
+    if s >= 9 and ra + rb - 2 * s <= 6:
+        assert prog.stats["splitk"] >= 1
+    assert got == pytest.approx(want, rel=tol, abs=tol * 1e-3 * abs(want))
+
+
+def test_geometry_is_numpy_order():
+    # output index = (left free index << b) | right free index
+    g = schedule.join_geometry(4, 3, [1, 3], [2, 0])
+    assert g["fx"] + g["fy"] == 3 and g["s"] == 2
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((2,) * 4)
+    b = rng.standard_normal((2,) * 3)
+    want = np.tensordot(a, b, ([1, 3], [2, 0])).ravel()
+    A, B = a.ravel(), b.ravel()
+    X, Y = (A, B) if g["x_is_left"] else (B, A)
+    out = np.zeros(8)
+    for f in range(1 << g["fx"]):
+        for y in range(1 << g["fy"]):
+            acc = 0.0
+            for k in range(1 << g["s"]):
+                xo = emulator.pdep_array(np.array([f]), g["xf"], g["fx"])[0] + \
+                    emulator.pdep_array(np.array([k]), g["xs"], g["s"])[0]
+                yo = emulator.pdep_array(np.array([y]), g["yf"], g["fy"])[0] + \
+                    emulator.pdep_array(np.array([k]), g["ys"], g["s"])[0]
+                acc += X[xo] * Y[yo]
+            out[(f << g["shf"]) | (y << g["shy"])] = acc
+    np.testing.assert_allclose(out, want, rtol=1e-13)
+
+
+def test_memory_plan_never_overlaps_live_buffers():
+    doc = json.load(open(os.path.join(HERE, "golden", "cnf40_factor_s6.json")))
+    plan = plan_io.plan_from_dict(doc)
+    tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
+    nodes = schedule.analyse(tree, tables)
+    for W in (1, 5, 64):
+        schedule.peak_bytes(nodes, tables, W, _lib.TO_F64)
+        live = {}
+        for n in nodes:
+            if n.is_leaf and not n.batched:
+                continue
+            if n.buf is None:
+                continue
+        # replay the wave order and check that each join's output does not overlap its
+        # operands or any other still-needed buffer
+        needed = {}
+        for n in nodes:
+            if not n.is_leaf:
+                for c in (n.left, n.right):
+                    needed[c] = n.idx
+        order = [n for n in nodes if not n.is_leaf and not n.batched] + \
+                [n for n in nodes if n.is_leaf and n.batched] + \
+                [n for n in nodes if not n.is_leaf and n.batched]
+        done = set()
+        for n in order:
+            if n.buf is None:
+                continue
+            s0, z0 = n.buf
+            for m in nodes:
+                if m.idx == n.idx or m.buf is None or m.idx not in done:
+                    continue
+                consumer = needed.get(m.idx)
+                still_needed = consumer is not None and consumer not in done
+                if still_needed:
+                    s1, z1 = m.buf
+                    assert s0 + z0 <= s1 or s1 + z1 <= s0, (n.idx, m.idx)
+            done.add(n.idx)

@@ -1,0 +1,163 @@
+"""GPU parity: the HIP executor against the oracle and the committed fixtures.
+
+Tolerances: plain FP64 executor rel <= 1e-12 of the FP64 oracle (exact for unweighted
+counts < 2^53); FP32-scaled executor rel <= 1e-6.
+"""
+
+import glob
+import itertools
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import numpy_oracle
+from tensororder_amd import plan_io
+from tensororder_amd.execution import run
+from tensororder_amd.plan import ALL_SLICERS
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "*.json")))
+SMALL = [f for f in FIXTURES if json.load(open(f))["meta"]["maxrank"] <= 20]
+F64_TOL = 1e-12
+F32_TOL = 1e-6
+
+
+def _name(path):
+    return os.path.basename(path)[:-5]
+
+
+def _say(*args):
+    print(*args, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from tensororder_amd.hip_api import HipAPI
+
+    return HipAPI()
+
+
+@pytest.fixture(scope="module")
+def hip32():
+    from tensororder_amd.hip_api import HipF32API
+
+    return HipF32API()
+
+
+def _expected(doc):
+    exp = doc["meta"]["expected"]
+    if "float64" in exp:
+        return exp["float64"]
+    return float(numpy_oracle.contract_sliced(doc))
+
+
+@pytest.mark.parametrize("path", SMALL, ids=[_name(p) for p in SMALL])
+def test_fixture_fp64(hip, path):
+    doc = json.load(open(path))
+    _say("case", _name(path), "fp64")
+    want = _expected(doc)
+    got = hip.contract_sliced(plan_io.plan_from_dict(doc))
+    assert got == pytest.approx(want, rel=F64_TOL)
+    if "bigint" in doc["meta"]["expected"]:
+        assert int(got) == int(doc["meta"]["expected"]["bigint"])
+
+
+@pytest.mark.parametrize("path", SMALL, ids=[_name(p) for p in SMALL])
+def test_fixture_fp32_scaled(hip32, path):
+    doc = json.load(open(path))
+    _say("case", _name(path), "fp32s")
+    want = _expected(doc)
+    got = hip32.contract_sliced(plan_io.plan_from_dict(doc))
+    assert got == pytest.approx(want, rel=F32_TOL)
+
+
+@pytest.mark.parametrize("W", [1, 3, 64])
+def test_batch_sizes_and_prefix(hip, W):
+    path = os.path.join(HERE, "golden", "cnf20_factor_s4.json")
+    doc = json.load(open(path))
+    _say("case batch", W)
+    hip.add_argument("max_batch", W)
+    try:
+        plan = plan_io.plan_from_dict(doc)
+        assert hip.contract_sliced(plan) == pytest.approx(_expected(doc), rel=F64_TOL)
+        for k in (1, 5, 16):
+            want = numpy_oracle.contract_sliced(doc, num_slice_limit=k)
+            assert hip.contract_sliced(plan, num_slice_limit=k) == pytest.approx(want, rel=F64_TOL)
+            assert hip.last_result.slices == k
+    finally:
+        hip.add_argument("max_batch", 1 << 16)
+
+
+def test_graph_and_eager_agree(hip):
+    doc = json.load(open(os.path.join(HERE, "golden", "cnf40_factor_s6.json")))
+    _say("case graph vs eager")
+    plan = plan_io.plan_from_dict(doc)
+    hip.add_argument("max_batch", 8)
+    try:
+        hip.add_argument("graph", True)
+        g = hip.contract_sliced(plan)
+        hip.add_argument("graph", False)
+        e = hip.contract_sliced(plan)
+    finally:
+        hip.add_argument("graph", True)
+        hip.add_argument("max_batch", 1 << 16)
+    assert g == e
+    assert g == pytest.approx(_expected(doc), rel=F64_TOL)
+
+
+def test_tensordot_matches_numpy(hip):
+    rng = np.random.default_rng(1)
+    _say("case tensordot")
+    for trial in range(40):
+        ra, rb = int(rng.integers(0, 9)), int(rng.integers(0, 9))
+        s = int(rng.integers(0, min(ra, rb) + 1))
+        la = [int(x) for x in rng.permutation(ra)[:s]]
+        lb = [int(x) for x in rng.permutation(rb)[:s]]
+        a = rng.standard_normal((2,) * ra)
+        b = rng.standard_normal((2,) * rb)
+        got = hip.tensordot(a, b, (la, lb)).numpy()
+        np.testing.assert_allclose(got, np.tensordot(a, b, (la, lb)), rtol=1e-12, atol=1e-12)
+
+
+def test_tensordot_large_shared_splitk(hip):
+    # few outputs, many shared bits: the split-K kernels
+    rng = np.random.default_rng(2)
+    _say("case splitk")
+    a = rng.standard_normal((2,) * 14)
+    b = rng.standard_normal((2,) * 13)
+    la = list(range(1, 14))
+    lb = list(rng.permutation(13))
+    got = hip.tensordot(a, b, (la, [int(x) for x in lb])).numpy()
+    want = np.tensordot(a, b, (la, [int(x) for x in lb]))
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-11)
+
+
+def test_permute_matches_numpy(hip):
+    rng = np.random.default_rng(3)
+    _say("case permute")
+    for r in (0, 1, 3, 7, 12, 16):
+        a = rng.standard_normal((2,) * r)
+        perm = [int(x) for x in rng.permutation(r)]
+        np.testing.assert_array_equal(hip.permute(a, perm), np.transpose(a, perm))
+
+
+def test_execution_run_slices_on_oom(hip):
+    # a device budget too small for the unsliced plan: the library raises
+    # OutOfMemoryError, execution.run slices once more and retries every slice
+    doc = json.load(open(os.path.join(HERE, "golden", "vc90_weighted_line.json")))
+    _say("case oom retry")
+    plan = plan_io.plan_from_dict(doc)
+    assert len(plan.groups_to_slice) == 0
+    hip.add_argument("mem_limit", 80 * 1024)
+    try:
+        got = run(plan, hip, ALL_SLICERS["greedy_mem"], None)
+    finally:
+        hip.add_argument("mem_limit", None)
+    assert len(plan.groups_to_slice) > 0
+    assert got == pytest.approx(_expected(doc), rel=F64_TOL)
