@@ -50,8 +50,11 @@ SPECS = {
                         ["float64"]),
     "vc90_weighted_line": (lambda: vertex_cover_cnf(90, 6, True), "cachet", "line", 5, {}, ["float64"]),
     "vc200_weighted_line": (lambda: vertex_cover_cnf(200, 0, True), "cachet", "line", 240, {}, ["float64"]),
-    "cnf120_factor_s12": (lambda: random_3cnf(120, 240, 0, True), "cachet", "factor", 300, {"slices": 12},
-                          []),
+    # the 120-variable / 240-clause instance of the original workload description has a
+    # factor-Flow plan of max rank 60 (treewidth 53): 12 slices leave rank 48, far beyond
+    # any memory.  70 variables at the same clause ratio gives rank 38 -> 25 with 12 slices.
+    "cnf70_factor_s12": (lambda: random_3cnf(70, 140, 0, True), "cachet", "factor", 120, {"slices": 12},
+                         ["prefix:2"]),
 }
 
 
@@ -103,6 +106,11 @@ def build(name, td_text=None):
         doc["meta"]["expected"]["float64"] = float(numpy_oracle.contract_sliced(doc))
     if "bigint" in expect:
         doc["meta"]["expected"]["bigint"] = str(numpy_oracle.contract_sliced(doc, "bigint"))
+    for e in expect:
+        if e.startswith("prefix:"):
+            k = int(e.split(":")[1])
+            doc["meta"]["expected"]["prefix"] = {
+                "slices": k, "float64": float(numpy_oracle.contract_sliced(doc, num_slice_limit=k))}
     if "brute" in expect:
         w = {v: formula._variables[v] for v in formula.variables}
         doc["meta"]["expected"]["brute"] = brute.brute_wmc(len(formula.variables), formula.clauses, w)

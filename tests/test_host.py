@@ -1,0 +1,261 @@
+"""Host logic: DIMACS parsing, network wiring, leaf values, join properties, cost model,
+slicers, the plan format and the execution driver's failure handling (CPU only)."""
+
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+import graphs
+from oracle import numpy_oracle
+from tensororder_amd import api, execution, network, plan_io, planners
+from tensororder_amd.decomposition import TreeDecomposition, min_degree_decomposition, parse_all
+from tensororder_amd.formula import Formula, WeightFormat, parse_dimacs_text
+from tensororder_amd.instances import random_3cnf, random_regular_graph, vertex_cover_cnf
+from tensororder_amd.plan import ALL_SLICERS, SlicedExecutionPlan
+from tensororder_amd.tree import TreeContext, is_tree_complete
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class NumpyOps:
+    """A minimal numpy tensor library, used to walk package structures on the CPU."""
+
+    def create_tensor(self, shape, default_value=None):
+        return np.full(shape, 0.0 if default_value is None else default_value, dtype=np.float64)
+
+    def tensordot(self, a, b, axes):
+        return np.tensordot(a, b, axes)
+
+
+# -- parsing ---------------------------------------------------------------------------
+
+def test_parse_weight_formats():
+    text = "c x\np cnf 4 2\nw 2 0.25\nw 3 -1\n1 -2 0\n3 4 0\n"
+    f = parse_dimacs_text(text, "cachet")
+    assert f.variables == [2, 3, 1, 4]  # w-line variables first, then the rest
+    assert f._variables[2] == [0.75, 0.25]
+    assert f._variables[3] == [1, 1]
+    assert f._variables[1] == [0.5, 0.5]
+    assert f.clauses == [[1, -2], [3, 4]]
+    g = parse_dimacs_text("p cnf 2 1\nw 1 0.3\nw -1 0.6\n1 2 0\n", "mcc")
+    assert g._variables[1] == [0.6, 0.3] and g._variables[2] == [1, 1]
+    h = parse_dimacs_text("p cnf 2 1\nc weights 0.4 0.6 0.7 0.3\n1 2 0\n", "minic2d")
+    assert h._variables[1] == [0.6, 0.4] and h._variables[2] == [0.3, 0.7]
+    with pytest.raises(RuntimeError):
+        parse_dimacs_text("p cnf 1 1\nw 1 0.5\n1 0\n", "unweighted")
+
+
+def test_random_regular_graph_is_simple_cubic():
+    rng = np.random.Generator(np.random.PCG64(0))
+    edges = random_regular_graph(50, 3, rng)
+    assert len(edges) == 75 and len(set(edges)) == 75
+    deg = np.bincount(np.array(edges).ravel(), minlength=50)
+    assert (deg == 3).all() and all(u < v for u, v in edges)
+
+
+# -- wiring and leaves -------------------------------------------------------------------
+
+def test_cnf_count_wiring():
+    f = parse_dimacs_text("p cnf 3 2\n1 -2 0\n2 3 -1 0\n", "unweighted")
+    net = network.cnf_count(f)
+    assert len(net) == 5  # 3 variables then 2 clauses
+    assert [net.tensor(i).rank for i in range(5)] == [2, 2, 1, 2, 3]
+    # the first clause meets each variable's last axis
+    assert net.index_list(0)[1] == net.index_list(3)[0]
+    assert net.index_list(1)[1] == net.index_list(3)[1]
+    assert net.index_list(2)[0] == net.index_list(4)[1]
+
+
+def test_leaf_builders_match_oracle():
+    ops = NumpyOps()
+    for rank in range(0, 5):
+        for pos in itertools.product((True, False), repeat=rank):
+            for o in [None] + list(range(rank)):
+                t = network.OrTensor(list(pos), o)
+                want = numpy_oracle.build_leaf({"kind": "or", "literals_positive": list(pos), "output_index": o})
+                np.testing.assert_array_equal(t.build(ops.create_tensor), want)
+        v = network.VariableTensor(rank, 0.8, 0.2)
+        want = numpy_oracle.build_leaf({"kind": "var", "rank": rank, "w_neg": 0.2, "w_pos": 0.8})
+        np.testing.assert_array_equal(v.build(ops.create_tensor), want)
+
+
+def test_slicing_keeps_unit_axes():
+    ops = NumpyOps()
+    v = network.VariableTensor(3, 0.9, 0.1)
+    s = v.get_slice(1, 1).get_slice(2, 1)
+    built = s.build(ops.create_tensor)
+    assert built.shape == (2, 1, 1)
+    assert built[1, 0, 0] == 0.9 and built[0, 0, 0] == 0.0
+
+
+def test_factor_components_contract_back():
+    # the two factored pieces joined over the new edge give back the original tensor
+    for pos in itertools.product((True, False), repeat=4):
+        for o in (None, 0, 3):
+            t = network.OrTensor(list(pos), o)
+            left, right = t.get_factor_components([0, 2], [1, 3])
+            ops = NumpyOps()
+            joined = np.tensordot(left.build(ops.create_tensor), right.build(ops.create_tensor), ([2], [2]))
+            full = t.build(ops.create_tensor).transpose(0, 2, 1, 3)
+            np.testing.assert_array_equal(joined, full)
+    v = network.VariableTensor(4, 0.7, 0.3)
+    left, right = v.get_factor_components([0, 1], [2, 3])
+    ops = NumpyOps()
+    joined = np.tensordot(left.build(ops.create_tensor), right.build(ops.create_tensor), ([2], [2]))
+    np.testing.assert_allclose(joined, v.build(ops.create_tensor))
+
+
+# -- trees, joins and the package walk ----------------------------------------------------
+
+def test_join_properties_numpy_order():
+    net = network.TensorNetwork()
+    for r in (4, 4):
+        net.add_node(network.BuiltTensor(np.zeros((2,) * r)))
+    net._index_lists = [[5, 1, 7, 3], [3, 9, 5, 2]]
+    ctx = TreeContext()
+    j = ctx.join(ctx.leaf(net, 0), ctx.leaf(net, 1))
+    n = ctx.nodes[j]
+    assert n.left_edge_map == [0, 3] and n.right_edge_map == [2, 0]
+    assert n.free_edges == [1, 7, 9, 2] and n.maxrank == 4
+
+
+@pytest.mark.parametrize("planner", ["line", "factor"])
+def test_package_walk_matches_oracle(planner):
+    f = parse_dimacs_text(vertex_cover_cnf(18, 2, True) if planner == "line" else random_3cnf(16, 40, 2),
+                          "cachet")
+    net = network.cnf_count(f)
+    plan = planners.best_plan(net, planner, solver=None, seed=1)
+    assert is_tree_complete(plan.tree, plan.network)
+    got = plan.network.identify(plan.tree, NumpyOps())[()]
+    want = numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan))
+    assert got == pytest.approx(want, rel=1e-13)
+    # the mirror's own slice loop (base_api) agrees as well
+    ALL_SLICERS["greedy_mem"].slice_until(plan, slices=3)
+
+    class Lib(api.BaseTensorAPI):
+        def contract(self, network_, tree):
+            return network_.identify(tree, NumpyOps())
+
+    assert Lib().contract_sliced(plan) == pytest.approx(want, rel=1e-13)
+
+
+def test_sort_small_puts_smaller_right():
+    f = parse_dimacs_text(random_3cnf(16, 40, 4), "cachet")
+    plan = planners.best_plan(network.cnf_count(f), "factor", solver=None, seed=0)
+    want = numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan))
+    plan.tree.sort_small()
+    for node in plan.tree.iterate_postorder():
+        if not node.is_leaf:
+            assert node.left.rank >= node.right.rank
+    assert plan.network.identify(plan.tree, NumpyOps())[()] == pytest.approx(want, rel=1e-13)
+
+
+# -- cost model and slicers ---------------------------------------------------------------
+
+def test_estimate_cost_small_chain():
+    # A(e0,e1) - B(e1,e2) - C(e2,e0): joins A,B then with C
+    net = network.TensorNetwork()
+    for _ in range(3):
+        net.add_node(network.BuiltTensor(np.ones((2, 2))))
+    net._index_lists = [[0, 1], [1, 2], [2, 0]]
+    net._edges = [[0, 0, 2], [1, 0, 1], [2, 1, 2]]
+    ctx = TreeContext()
+    root = ctx.join(ctx.join(ctx.leaf(net, 0), ctx.leaf(net, 1)), ctx.leaf(net, 2))
+    flops, mem, best, maxrank = ctx.get_tree(root).estimate_cost()
+    # AB: 2^(1 + 2) terms; (AB)C: 2^(2 + 0)
+    assert flops == 8 + 4
+    assert maxrank == 2
+    # memory cap: result + 2*(left + right) at the first join = 4 + 2*(4 + 4)
+    assert mem == 20
+    flops2, mem2, _, maxrank2 = ctx.get_tree(root).estimate_cost({1})
+    assert maxrank2 == 2 and mem2 < mem
+
+
+def test_slicers_reach_limits_and_groups_are_equivalence_classes():
+    f = parse_dimacs_text(random_3cnf(30, 70, 6), "cachet")
+    plan = planners.best_plan(network.cnf_count(f), "factor", solver=None, seed=0)
+    r0 = plan.maxrank
+    ALL_SLICERS["greedy_mem"].slice_until(plan, rank=r0 - 3)
+    assert plan.maxrank <= r0 - 3
+    for g in plan.groups_to_slice:
+        e = next(iter(g))
+        assert plan.network.find_equivalent_edges(e) == g
+    plan2 = planners.best_plan(network.cnf_count(f), "factor", solver=None, seed=0)
+    ALL_SLICERS["greedy_most"].slice_until(plan2, rank=r0 - 3)
+    assert plan2.maxrank <= r0 - 3
+    plan3 = planners.best_plan(network.cnf_count(f), "factor", solver=None, seed=0)
+    with pytest.raises(RuntimeError):
+        ALL_SLICERS["disable"].slice_until(plan3, rank=r0 - 1)
+
+
+# -- plan format ----------------------------------------------------------------------------
+
+def test_plan_roundtrip_and_fixtures():
+    f = parse_dimacs_text(random_3cnf(20, 50, 8), "cachet")
+    plan = planners.best_plan(network.cnf_count(f), "factor", solver=None, seed=0)
+    ALL_SLICERS["greedy_mem"].slice_until(plan, slices=2)
+    d = plan_io.plan_to_dict(plan)
+    back = plan_io.plan_from_dict(json.loads(json.dumps(d)))
+    assert plan_io.plan_to_dict(back) == d
+    assert back.maxrank == plan.maxrank and back.memory == plan.memory
+    for path in sorted(os.listdir(os.path.join(HERE, "golden"))):
+        if not path.endswith(".json"):
+            continue
+        doc = json.load(open(os.path.join(HERE, "golden", path)))
+        p = plan_io.plan_from_dict(doc)
+        assert p.maxrank == doc["meta"]["maxrank"]
+        assert is_tree_complete(p.tree, p.network)
+
+
+def test_decomposition_text_roundtrip():
+    td = min_degree_decomposition(6, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (5, 0), (0, 3)])
+    again = parse_all(td.to_text(6))[0]
+    assert again.bags == td.bags
+    assert sorted(map(sorted, again.neighbors)) == sorted(map(sorted, td.neighbors))
+    # every graph edge is inside a bag
+    for u, v in [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (5, 0), (0, 3)]:
+        assert any(u in b and v in b for b in td.bags)
+
+
+# -- execution driver failure handling ---------------------------------------------------
+
+class FlakyAPI(api.BaseTensorAPI):
+    def __init__(self, fail_times, exc=api.OutOfMemoryError):
+        self.fail_times = fail_times
+        self.exc = exc
+        self.calls = 0
+
+    def contract_sliced(self, plan, num_slice_limit=None):
+        self.calls += 1
+        if self.calls <= self.fail_times:
+            raise self.exc
+        return 42.0
+
+
+def _plan():
+    f = parse_dimacs_text(random_3cnf(20, 50, 9), "cachet")
+    return planners.best_plan(network.cnf_count(f), "factor", solver=None, seed=0)
+
+
+def test_run_slices_once_per_oom():
+    plan = _plan()
+    lib = FlakyAPI(3)
+    assert execution.run(plan, lib, ALL_SLICERS["greedy_mem"], None, log=lambda k, v: None) == 42.0
+    assert len(plan.groups_to_slice) == 3 and lib.calls == 4
+
+
+def test_run_reports_errors_as_none():
+    seen = []
+    for exc, msg in ((TimeoutError, "Timeout during execution"), (MemoryError, "Out of Memory during execution"),
+                     (ValueError, "Exception during execution")):
+        out = execution.run(_plan(), FlakyAPI(1, exc), ALL_SLICERS["greedy_mem"], None,
+                            log=lambda k, v: seen.append((k, v)))
+        assert out is None and seen[-1] == ("Error", msg)
+
+
+def test_base_api_rejects_unknown_arguments():
+    with pytest.raises(ValueError):
+        api.BaseTensorAPI().add_argument("nope", 1)
