@@ -54,6 +54,11 @@ extern "C" {
                                 aux2: to_accum*, aux3: wave params (int64[2]) */
 #define TO_OP_WAVE_ADVANCE 7 /* aux3: wave params; adds iaux0 to params[0] */
 #define TO_OP_STEP_BEGIN 8   /* aux2: to_accum* (zeroed), aux3: wave params (params[0] = 0) */
+#define TO_OP_PERMUTE 9      /* descs: to_perm_desc[count] */
+#define TO_OP_GEMM 10        /* descs: to_gemm_desc[count]; variant 0: 64x64 output tiles
+                                (mlog, nlog >= 6), variant 1: 128x128 tiles (mlog, nlog >= 7) */
+#define TO_OP_ROOT_STORE 11  /* descs: to_root_desc[count]: one instance's root -> its accumulator */
+#define TO_OP_BATCH_BEGIN 12 /* aux2: to_accum[iaux0] zeroed; aux3: int64 start time (device clock) */
 
 #define TO_MAX_BITS 32
 
@@ -105,6 +110,46 @@ typedef struct to_accum {
     int64_t pad;
 } to_accum;
 
+/* A bit permutation of a batch of rank-r tensors: input bit p goes to output bit
+ * out_pos(p).  The tile is the union of the lowest input and output bits (u: input
+ * order, v: output order); the remaining bits enumerate tiles. */
+typedef struct to_perm_desc {
+    uint64_t in, out;
+    int64_t inb, outb;   /* batch strides in elements */
+    int32_t rank, nu, nrest, nbatch;
+    int32_t block0, nblocks;
+    int8_t u_in[16];     /* input bit of tile bit j (u order) */
+    int8_t v_out[16];    /* output bit of tile bit j (v order) */
+    int8_t v2u[16];      /* tile bit j of v order is tile bit v2u[j] of u order */
+    int8_t rest_in[32];  /* input bit of rest bit j */
+    int8_t rest_out[32]; /* output bit of rest bit j */
+} to_perm_desc;
+
+/* A batched dense product C[m][n] = sum_k A[m][k] * B[n][k]: A is 2^mlog x 2^klog and
+ * B is 2^nlog x 2^klog, both row-major with k contiguous; C entry (m, n) is stored at
+ * c + batch*cb + m*ldcm + n*ldcn.  Requires mlog, nlog >= 6 (>= 7 for 128x128 tiles)
+ * and klog >= 4 (>= 5 for float with 128x128 tiles). */
+typedef struct to_gemm_desc {
+    uint64_t a, b, c;
+    int64_t ab, bb, cb;        /* batch strides in elements (0 = broadcast) */
+    int64_t ldcm, ldcn;
+    uint64_t as, bs, cs;       /* to_scale_rec* of the operands and result (TO_F32S) */
+    int64_t asb, bsb;          /* scale-record batch strides */
+    int32_t mlog, nlog, klog;
+    int32_t nbatch;
+    int32_t block0, nblocks;
+    int32_t pad0, pad1;
+} to_gemm_desc;
+
+/* The root scalar of one instance of a batch: stored into its accumulator as
+ * (mantissa, exponent), with the device clock at that moment in accum.pad. */
+typedef struct to_root_desc {
+    uint64_t root; /* the rank-0 result */
+    uint64_t rec;  /* its to_scale_rec (TO_F32S) or 0 */
+    uint64_t acc;  /* to_accum* */
+    int32_t block0, nblocks;
+} to_root_desc;
+
 /* One kernel launch. */
 typedef struct to_launch {
     int32_t op;
@@ -123,7 +168,8 @@ int to_version(void);
 
 /* Size in bytes of each record type, for host-side layout checks. */
 int to_struct_sizes(int64_t* join_desc, int64_t* leaf_desc, int64_t* scale_rec,
-                     int64_t* accum, int64_t* launch);
+                     int64_t* accum, int64_t* launch, int64_t* perm_desc, int64_t* gemm_desc,
+                     int64_t* root_desc);
 
 /* Last error message of the calling thread ("" when none). */
 const char* to_last_error(void);
@@ -154,6 +200,9 @@ int to_graph_destroy(void* graph_exec);
  * tensor.py:112-117).  Element size 4 or 8 bytes. */
 int to_permute(const void* in, void* out, int rank, const int32_t* perm, int elem_bytes,
                int64_t batch, void* stream);
+
+/* Frequency of the device clock stamped by TO_OP_ROOT_STORE / TO_OP_BATCH_BEGIN, in kHz. */
+int to_wall_clock_khz(int* khz);
 
 /* Block until all work on `stream` is done. */
 int to_synchronize(void* stream);

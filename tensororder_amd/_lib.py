@@ -21,9 +21,13 @@ TO_F64, TO_F32S = 0, 1
     TO_OP_ROOT_REDUCE,
     TO_OP_WAVE_ADVANCE,
     TO_OP_STEP_BEGIN,
-) = range(1, 9)
+    TO_OP_PERMUTE,
+    TO_OP_GEMM,
+    TO_OP_ROOT_STORE,
+    TO_OP_BATCH_BEGIN,
+) = range(1, 13)
 MAX_BITS = 32
-ABI_VERSION = 1
+ABI_VERSION = 3
 
 c_i8x32 = ctypes.c_int8 * MAX_BITS
 
@@ -64,6 +68,38 @@ class Accum(ctypes.Structure):
                 ("count", ctypes.c_int64), ("pad", ctypes.c_int64)]
 
 
+c_i8x16 = ctypes.c_int8 * 16
+
+
+class PermDesc(ctypes.Structure):
+    _fields_ = [
+        ("in_", ctypes.c_uint64), ("out", ctypes.c_uint64),
+        ("inb", ctypes.c_int64), ("outb", ctypes.c_int64),
+        ("rank", ctypes.c_int32), ("nu", ctypes.c_int32), ("nrest", ctypes.c_int32), ("nbatch", ctypes.c_int32),
+        ("block0", ctypes.c_int32), ("nblocks", ctypes.c_int32),
+        ("u_in", c_i8x16), ("v_out", c_i8x16), ("v2u", c_i8x16),
+        ("rest_in", c_i8x32), ("rest_out", c_i8x32),
+    ]
+
+
+class GemmDesc(ctypes.Structure):
+    _fields_ = [
+        ("a", ctypes.c_uint64), ("b", ctypes.c_uint64), ("c", ctypes.c_uint64),
+        ("ab", ctypes.c_int64), ("bb", ctypes.c_int64), ("cb", ctypes.c_int64),
+        ("ldcm", ctypes.c_int64), ("ldcn", ctypes.c_int64),
+        ("as_", ctypes.c_uint64), ("bs", ctypes.c_uint64), ("cs", ctypes.c_uint64),
+        ("asb", ctypes.c_int64), ("bsb", ctypes.c_int64),
+        ("mlog", ctypes.c_int32), ("nlog", ctypes.c_int32), ("klog", ctypes.c_int32),
+        ("nbatch", ctypes.c_int32), ("block0", ctypes.c_int32), ("nblocks", ctypes.c_int32),
+        ("pad0", ctypes.c_int32), ("pad1", ctypes.c_int32),
+    ]
+
+
+class RootDesc(ctypes.Structure):
+    _fields_ = [("root", ctypes.c_uint64), ("rec", ctypes.c_uint64), ("acc", ctypes.c_uint64),
+                ("block0", ctypes.c_int32), ("nblocks", ctypes.c_int32)]
+
+
 class Launch(ctypes.Structure):
     _fields_ = [
         ("op", ctypes.c_int32), ("variant", ctypes.c_int32), ("count", ctypes.c_int32),
@@ -78,7 +114,7 @@ class Launch(ctypes.Structure):
 EXPORTED = [
     "to_version", "to_struct_sizes", "to_last_error", "to_device_count", "to_set_device",
     "to_run", "to_run_timed", "to_graph_create", "to_graph_launch", "to_graph_destroy", "to_permute",
-    "to_synchronize",
+    "to_wall_clock_khz", "to_synchronize",
 ]
 
 _lib = None
@@ -101,7 +137,8 @@ def load(path=LIB_PATH):
     vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
     p64 = ctypes.POINTER(ctypes.c_int64)
     lib.to_version.restype = i32
-    lib.to_struct_sizes.argtypes = [p64] * 5
+    lib.to_struct_sizes.argtypes = [p64] * 8
+    lib.to_wall_clock_khz.argtypes = [ctypes.POINTER(ctypes.c_int)]
     lib.to_last_error.restype = ctypes.c_char_p
     lib.to_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     lib.to_set_device.argtypes = [i32]
@@ -114,9 +151,9 @@ def load(path=LIB_PATH):
     lib.to_synchronize.argtypes = [vp]
     if lib.to_version() != ABI_VERSION:
         raise ImportError("libtensororder_hip.so ABI version mismatch; rebuild it")
-    sizes = [ctypes.c_int64() for _ in range(5)]
+    sizes = [ctypes.c_int64() for _ in range(8)]
     lib.to_struct_sizes(*[ctypes.byref(s) for s in sizes])
-    want = [ctypes.sizeof(t) for t in (JoinDesc, LeafDesc, ScaleRec, Accum, Launch)]
+    want = [ctypes.sizeof(t) for t in (JoinDesc, LeafDesc, ScaleRec, Accum, Launch, PermDesc, GemmDesc, RootDesc)]
     if [s.value for s in sizes] != want:
         raise ImportError("record layout mismatch between library and binding: %s vs %s"
                           % ([s.value for s in sizes], want))

@@ -1,0 +1,188 @@
+"""Many independent instances in one schedule (a batch of unsliced plans of uneven size).
+
+Each instance keeps its own compiled post-order join list and its own arena region.  Step
+k of the batch runs the k-th join of every instance that has one, as grouped launches (one
+per kernel variant) whose blocks locate their join by binary search over ``block0`` --
+so a batch of N instances needs max_i(#joins_i) steps rather than sum_i(#joins_i)
+launches, and independent small joins of different instances share the machine.  Right
+after its last join an instance's root is stored into its own accumulator together with
+the device clock, which gives per-instance completion times within the batch.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib, schedule
+from ._lib import Launch
+
+ALIGN = schedule.ALIGN
+KIND_ORDER = {"permute": 0, "stream": 1, "splitk": 1, "gemm": 1, "finish": 2}
+
+
+class BatchProgram:
+    def __init__(self):
+        self.launches = []
+        self.info = []
+        self.count = 0
+        self.stats = {}
+
+
+def compile_batch(instances, entry_type, device_alloc, regroup=True):
+    """``instances``: list of (tree, tables) of unsliced plans.  Returns a BatchProgram."""
+    esize = 8 if entry_type == _lib.TO_F64 else 4
+    scaled = entry_type == _lib.TO_F32S
+    fdtype = np.float64 if esize == 8 else np.float32
+    per = []
+    arena_top = 0
+    for tree, tables in instances:
+        nodes = schedule.analyse(tree, tables, regroup)
+        if any(n.batched for n in nodes):
+            raise ValueError("batched instances must be unsliced plans")
+        for n in nodes:
+            if not n.is_leaf:
+                g = schedule.join_geometry(nodes[n.left].rank, nodes[n.right].rank, n.lmap, n.rmap)
+                g["var"] = schedule.choose_variant(g, 1, esize)
+                n.geom = g
+        peak, scratch = schedule.plan_memory(nodes, tables, 1, esize)
+        per.append(dict(nodes=nodes, tables=tables, base=arena_top, scratch=scratch, peak=peak))
+        arena_top += (max(peak, ALIGN) + ALIGN - 1) // ALIGN * ALIGN
+
+    prog = BatchProgram()
+    prog.count = len(instances)
+    # leaf tables
+    chunks, off = [], 0
+    for inst in per:
+        inst["table_off"], inst["exp0"] = {}, {}
+        for n in inst["nodes"]:
+            if n.is_leaf and n.tensor not in inst["table_off"]:
+                rows = inst["tables"][n.tensor].rows
+                if scaled:
+                    m = float(np.max(np.abs(rows))) if rows.size else 0.0
+                    e = math.frexp(m)[1] if m > 0 else 0
+                    inst["exp0"][n.tensor] = e
+                    rows = np.ldexp(rows, -e)
+                arr = np.ascontiguousarray(rows[:1], dtype=fdtype).ravel()
+                inst["table_off"][n.tensor] = off
+                chunks.append((off, arr))
+                off += (arr.size * esize + ALIGN - 1) // ALIGN * ALIGN // esize
+    table_host = np.zeros(max(off, 1), dtype=fdtype)
+    for o, arr in chunks:
+        table_host[o:o + arr.size] = arr
+    # scale records: constant leaves first (initialised once), then joins (cleared per run)
+    nrec, rec_init = 0, []
+    if scaled:
+        for inst in per:
+            for n in inst["nodes"]:
+                if n.is_leaf:
+                    n.rec, n.rec_stride = nrec, 0
+                    rec_init.append((nrec, inst["exp0"][n.tensor]))
+                    nrec += 1
+        join_rec0 = nrec
+        for inst in per:
+            for n in inst["nodes"]:
+                if not n.is_leaf:
+                    n.rec, n.rec_stride = nrec, 0
+                    nrec += 1
+    prog.table = device_alloc(table_host.nbytes)
+    prog.arena = device_alloc(max(ALIGN, arena_top))
+    prog.recs = device_alloc(max(1, nrec) * ctypes.sizeof(_lib.ScaleRec))
+    prog.accum = device_alloc(len(per) * ctypes.sizeof(_lib.Accum))
+    prog.start = device_alloc(64)
+    prog.table_host = table_host
+    prog.rec_init = rec_init
+    prog.rec_count = nrec
+    tbase, abase, rbase = prog.table.data_ptr(), prog.arena.data_ptr(), prog.recs.data_ptr()
+    rec_size = ctypes.sizeof(_lib.ScaleRec)
+    acc_size = ctypes.sizeof(_lib.Accum)
+
+    def recp(n):
+        return rbase + n.rec * rec_size if scaled else 0
+
+    for inst in per:
+        for n in inst["nodes"]:
+            if n.is_leaf:
+                n.ptr = tbase + inst["table_off"][n.tensor] * esize
+            else:
+                n.ptr = abase + inst["base"] + n.buf[0]
+            n.bstride = 0
+
+    # group every instance's k-th join into step k
+    steps = {}
+    roots = {}
+    for i, inst in enumerate(per):
+        nodes = inst["nodes"]
+        joins = [n for n in nodes if not n.is_leaf]
+        scratch = {k: {kk: (inst["base"] + blk[0], blk[1]) for kk, blk in v.items()}
+                   for k, v in inst["scratch"].items()}
+        for k, n in enumerate(joins):
+            for kind, op, variant, lds, st, nblk in schedule.join_records(
+                    n, nodes, 1, esize, scaled, abase, scratch.get(n.idx, {}), recp):
+                key = (KIND_ORDER[kind], op, variant)
+                steps.setdefault(k, {}).setdefault(key, []).append((st, nblk, lds, kind, i))
+        root = nodes[-1]
+        rd = _lib.RootDesc(root=root.ptr, rec=recp(root), acc=prog.accum.data_ptr() + i * acc_size)
+        roots.setdefault(len(joins), []).append(rd)
+
+    blob = bytearray()
+
+    def put(structs):
+        nonlocal blob
+        start = (len(blob) + ALIGN - 1) // ALIGN * ALIGN
+        blob += b"\0" * (start - len(blob))
+        for st in structs:
+            blob += bytes(st)
+        return start
+
+    pending = []
+    pending.append((Launch(op=_lib.TO_OP_BATCH_BEGIN, grid=max(1, min(64, -(-len(per) // 256))),
+                           aux2=prog.accum.data_ptr(), aux3=prog.start.data_ptr(), iaux0=len(per)), None,
+                    ("begin", -1)))
+    if scaled and nrec > join_rec0:
+        pending.append((Launch(op=_lib.TO_OP_SCALE_CLEAR, grid=max(1, min(1024, -(-(nrec - join_rec0) // 256))),
+                               aux0=rbase + join_rec0 * rec_size, iaux0=nrec - join_rec0), None, ("clear", -1)))
+    nsteps = max(list(steps) + list(roots)) + 1 if (steps or roots) else 0
+    launches_per_kind = {}
+    for k in range(nsteps):
+        for key in sorted(steps.get(k, {})):
+            group = steps[k][key]
+            structs, block0, lds = [], 0, 0
+            for st, nblk, l, kind, i in group:
+                st.block0 = block0
+                block0 += nblk
+                lds = max(lds, l)
+                structs.append(st)
+            off = put(structs)
+            kind = group[0][3]
+            pending.append((Launch(op=key[1], variant=key[2], count=len(structs), grid=block0, lds=lds), off,
+                            (kind, k)))
+            launches_per_kind[kind] = launches_per_kind.get(kind, 0) + 1
+        if k in roots:
+            rs = roots[k]
+            off = put(rs)
+            pending.append((Launch(op=_lib.TO_OP_ROOT_STORE, count=len(rs), grid=max(1, -(-len(rs) // 256))),
+                            off, ("root", k)))
+    prog.blob_host = np.frombuffer(bytes(blob) or b"\0", dtype=np.uint8).copy()
+    prog.blob = device_alloc(prog.blob_host.nbytes)
+    bbase = prog.blob.data_ptr()
+    for L, off, info in pending:
+        if off is not None:
+            L.descs = bbase + off
+        prog.launches.append(L)
+        prog.info.append(info)
+    prog.per = per
+    prog.stats = dict(instances=len(per), steps=nsteps, launches=len(prog.launches),
+                      joins=sum(sum(1 for n in p["nodes"] if not n.is_leaf) for p in per),
+                      arena_bytes=arena_top, launches_per_kind=launches_per_kind)
+    return prog
+
+
+def rec_init_bytes(prog):
+    n = max(1, prog.rec_count)
+    recs = (_lib.ScaleRec * n)()
+    one = int(np.array([1.0], dtype=np.float32).view(np.uint32)[0])
+    for i, e in prog.rec_init:
+        recs[i].exp = e
+        recs[i].maxbits = one
+    return bytes(recs)

@@ -23,7 +23,7 @@
 
 #include "tensororder_hip.h"
 
-#define TO_ABI_VERSION 1
+#define TO_ABI_VERSION 3
 
 namespace {
 
@@ -382,6 +382,33 @@ __global__ void k_wave_advance(int64_t* wave, int64_t step) {
     if (threadIdx.x == 0 && blockIdx.x == 0) wave[0] += step;
 }
 
+// Store each listed instance's root value into its accumulator, stamping the device clock.
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_root_store(const to_root_desc* __restrict__ descs, int ndesc) {
+    const int64_t now = (int64_t)wall_clock64();
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < ndesc; i += gridDim.x * kBlock) {
+        const to_root_desc& d = descs[i];
+        const double v = (double)*reinterpret_cast<const T*>(d.root);
+        int e = 0;
+        const double m = frexp(v, &e);
+        to_accum* acc = reinterpret_cast<to_accum*>(d.acc);
+        acc->mant = m;
+        acc->exp = m == 0.0 ? 0 : (int64_t)e + (SCALED ? reinterpret_cast<const to_scale_rec*>(d.rec)->exp : 0);
+        acc->count = 1;
+        acc->pad = now;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_begin(to_accum* acc, int64_t n, int64_t* start) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        acc[i].mant = 0.0;
+        acc[i].exp = 0;
+        acc[i].count = 0;
+        acc[i].pad = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && start) *start = (int64_t)wall_clock64();
+}
+
 __global__ void k_step_begin(to_accum* acc, int64_t* wave) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         acc->mant = 0.0;
@@ -503,6 +530,254 @@ __global__ __launch_bounds__(kBlock) void k_permute(const T* __restrict__ in, T*
 }
 
 // ---------------------------------------------------------------------------------
+// descriptor-driven bit permutation (same tiling as k_permute), used to bring the two
+// operands of a dense join into row-major [free][shared] form for the MFMA product
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute_desc(const to_perm_desc* __restrict__ descs, int ndesc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* tile = reinterpret_cast<T*>(smem);
+    const to_perm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int U = 1 << d->nu;
+    const int nrest = d->nrest;
+    const int64_t ntiles = (int64_t)d->nbatch << nrest;
+    const T* in = reinterpret_cast<const T*>(d->in);
+    T* out = reinterpret_cast<T*>(d->out);
+    for (int64_t t = blockIdx.x - d->block0; t < ntiles; t += d->nblocks) {
+        const int64_t b = t >> nrest;
+        const int64_t r = t & ((1ll << nrest) - 1);
+        const int64_t ibase = b * d->inb + pdep(r, d->rest_in, nrest);
+        const int64_t obase = b * d->outb + pdep(r, d->rest_out, nrest);
+        __syncthreads();
+        for (int u = threadIdx.x; u < U; u += kBlock) tile[u] = in[ibase + pdep(u, d->u_in, d->nu)];
+        __syncthreads();
+        for (int v = threadIdx.x; v < U; v += kBlock) {
+            int u = 0;
+            for (int j = 0; j < d->nu; ++j) u |= ((v >> j) & 1) << d->v2u[j];
+            out[obase + pdep(v, d->v_out, d->nu)] = tile[u];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// MFMA product of a dense join: C[m][n] = sum_k A[m][k] * B[n][k] on 64x64 output tiles.
+// 4 waves per block, each owning a 32x32 quarter as 2x2 MFMA 16x16 tiles; K advances 16 at
+// a time through LDS (k-major, so the 16 lanes reading one MFMA operand row hit 16
+// consecutive LDS words).  Operand layout of the 16x16x4 instructions: A[i][k] in lane
+// i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i/4), element i%4.
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+    typedef double V __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V op(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Mfma<float> {
+    typedef float V __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V op(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict__ descs, int ndesc) {
+    constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
+    __shared__ T As[BK][BM + PAD];
+    __shared__ T Bs[BK][BN + PAD];
+    typedef typename Mfma<T>::V V;
+    const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t K = 1ll << d->klog;
+    const int64_t tiles_m = 1ll << (d->mlog - 6), tiles_n = 1ll << (d->nlog - 6);
+    const int64_t per_batch = tiles_m * tiles_n;
+    const int64_t ntiles = (int64_t)d->nbatch * per_batch;
+    const int lrow = tid >> 2, lk = (tid & 3) * 4;
+    for (int64_t t = blockIdx.x - d->block0; t < ntiles; t += d->nblocks) {
+        const int64_t batch = t / per_batch;
+        const int64_t rem = t - batch * per_batch;
+        const int64_t tm = rem / tiles_n, tn = rem - (rem / tiles_n) * tiles_n;
+        const T* A = reinterpret_cast<const T*>(d->a) + batch * d->ab + (tm * BM + lrow) * K + lk;
+        const T* B = reinterpret_cast<const T*>(d->b) + batch * d->bb + (tn * BN + lrow) * K + lk;
+        T sa = T(1), sb = T(1);
+        int64_t ea = 0, eb = 0;
+        if (SCALED) {
+            const OpScale x = op_scale(d->as, d->asb, batch), y = op_scale(d->bs, d->bsb, batch);
+            sa = T(x.f); ea = x.e; sb = T(y.f); eb = y.e;
+        }
+        V acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[i][j][v] = T(0);
+        for (int64_t k0 = 0; k0 < K; k0 += BK) {
+            T ra[4], rb[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                ra[j] = A[k0 + j] * sa;
+                rb[j] = B[k0 + j] * sb;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                As[lk + j][lrow] = ra[j];
+                Bs[lk + j][lrow] = rb[j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < BK / 4; ++kk) {
+                const int kr = kk * 4 + (lane >> 4);
+                T a[2], b[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) a[i] = As[kr][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) b[j] = Bs[kr][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
+            }
+        }
+        T* C = reinterpret_cast<T*>(d->c) + batch * d->cb;
+        float mx = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int64_t n = tn * BN + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int64_t m = tm * BM + wm * 32 + i * 16 + (lane >> 4) * 4 + v;
+                    C[m * d->ldcm + n * d->ldcn] = acc[i][j][v];
+                    if (SCALED) mx = fmaxf(mx, (float)absval(acc[i][j][v]));
+                }
+            }
+        }
+        if (SCALED) {
+            mx = wave_max(mx);
+            if (lane == 0) note_max(d->cs, batch, mx);
+            if (tid == 0 && tm == 0 && tn == 0) note_exp(d->cs, batch, ea + eb);
+        }
+    }
+}
+
+// 128x128 output tiles: each wave owns a 64x64 quarter as 4x4 MFMA tiles; K advances by
+// BK (16 for double, 32 for float) with 16-byte global loads, and the next K slab is
+// fetched into registers while the current one is multiplied out of LDS.
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restrict__ descs, int ndesc) {
+    constexpr int BM = 128, BN = 128, PAD = 4;
+    constexpr int VE = 16 / sizeof(T);              // elements per 16-byte load
+    constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+    constexpr int VPR = BK / VE;                     // 16-byte vectors per tile row
+    constexpr int NV = BM * VPR / kBlock;            // vectors per thread per operand (4)
+    __shared__ T As[BK][BM + PAD];
+    __shared__ T Bs[BK][BN + PAD];
+    typedef typename Mfma<T>::V V;
+    typedef T L __attribute__((ext_vector_type(VE)));
+    const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t K = 1ll << d->klog;
+    const int64_t tiles_m = 1ll << (d->mlog - 7), tiles_n = 1ll << (d->nlog - 7);
+    const int64_t per_batch = tiles_m * tiles_n;
+    const int64_t ntiles = (int64_t)d->nbatch * per_batch;
+    int vrow[NV], vk[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int v = tid + kBlock * i;
+        vrow[i] = v / VPR;
+        vk[i] = (v % VPR) * VE;
+    }
+    for (int64_t t = blockIdx.x - d->block0; t < ntiles; t += d->nblocks) {
+        const int64_t batch = t / per_batch;
+        const int64_t rem = t - batch * per_batch;
+        const int64_t tm = rem / tiles_n, tn = rem - (rem / tiles_n) * tiles_n;
+        const T* A = reinterpret_cast<const T*>(d->a) + batch * d->ab + tm * BM * K;
+        const T* B = reinterpret_cast<const T*>(d->b) + batch * d->bb + tn * BN * K;
+        T sa = T(1), sb = T(1);
+        int64_t ea = 0, eb = 0;
+        if (SCALED) {
+            const OpScale x = op_scale(d->as, d->asb, batch), y = op_scale(d->bs, d->bsb, batch);
+            sa = T(x.f); ea = x.e; sb = T(y.f); eb = y.e;
+        }
+        V acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[i][j][v] = T(0);
+        L ra[NV], rb[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            ra[i] = *reinterpret_cast<const L*>(A + vrow[i] * K + vk[i]);
+            rb[i] = *reinterpret_cast<const L*>(B + vrow[i] * K + vk[i]);
+        }
+        for (int64_t k0 = 0; k0 < K; k0 += BK) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+#pragma unroll
+                for (int e = 0; e < VE; ++e) {
+                    As[vk[i] + e][vrow[i]] = ra[i][e] * sa;
+                    Bs[vk[i] + e][vrow[i]] = rb[i][e] * sb;
+                }
+            }
+            __syncthreads();
+            if (k0 + BK < K) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) {
+                    ra[i] = *reinterpret_cast<const L*>(A + vrow[i] * K + k0 + BK + vk[i]);
+                    rb[i] = *reinterpret_cast<const L*>(B + vrow[i] * K + k0 + BK + vk[i]);
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < BK / 4; ++kk) {
+                const int kr = kk * 4 + (lane >> 4);
+                T a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) b[j] = Bs[kr][wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
+            }
+        }
+        T* C = reinterpret_cast<T*>(d->c) + batch * d->cb;
+        float mx = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int64_t n = tn * BN + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int64_t m = tm * BM + wm * 64 + i * 16 + (lane >> 4) * 4 + v;
+                    C[m * d->ldcm + n * d->ldcn] = acc[i][j][v];
+                    if (SCALED) mx = fmaxf(mx, (float)absval(acc[i][j][v]));
+                }
+            }
+        }
+        if (SCALED) {
+            mx = wave_max(mx);
+            if (lane == 0) note_max(d->cs, batch, mx);
+            if (tid == 0 && tm == 0 && tn == 0) note_exp(d->cs, batch, ea + eb);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // host dispatch
 
 int grid_ok(const to_launch& L) {
@@ -569,6 +844,34 @@ int launch_one(const to_launch& L, hipStream_t st) {
             hipLaunchKernelGGL(k_wave_advance, dim3(1), dim3(64), 0, st, reinterpret_cast<int64_t*>(L.aux3),
                                L.iaux0);
             break;
+        case TO_OP_PERMUTE:
+            if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
+            hipLaunchKernelGGL((k_permute_desc<T>), grid, block, L.lds, st,
+                               reinterpret_cast<const to_perm_desc*>(L.descs), L.count);
+            break;
+        case TO_OP_GEMM:
+            if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
+            if (L.variant == 1)
+                hipLaunchKernelGGL((k_gemm128<T, SCALED>), grid, block, 0, st,
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+            else if (L.variant == 0)
+                hipLaunchKernelGGL((k_gemm<T, SCALED>), grid, block, 0, st,
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+            else {
+                set_err("gemm: unknown tile variant");
+                return TO_ERR_INVALID;
+            }
+            break;
+        case TO_OP_ROOT_STORE:
+            if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
+            hipLaunchKernelGGL((k_root_store<T, SCALED>), grid, block, 0, st,
+                               reinterpret_cast<const to_root_desc*>(L.descs), L.count);
+            break;
+        case TO_OP_BATCH_BEGIN:
+            if (!L.aux2 || L.iaux0 <= 0) return TO_ERR_INVALID;
+            hipLaunchKernelGGL(k_batch_begin, grid, block, 0, st, reinterpret_cast<to_accum*>(L.aux2), L.iaux0,
+                               reinterpret_cast<int64_t*>(L.aux3));
+            break;
         case TO_OP_STEP_BEGIN:
             if (!L.aux2 || !L.aux3) return TO_ERR_INVALID;
             hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, st, reinterpret_cast<to_accum*>(L.aux2),
@@ -617,7 +920,10 @@ extern "C" {
 int to_version(void) { return TO_ABI_VERSION; }
 
 int to_struct_sizes(int64_t* join_desc, int64_t* leaf_desc, int64_t* scale_rec, int64_t* accum,
-                    int64_t* launch) {
+                    int64_t* launch, int64_t* perm_desc, int64_t* gemm_desc, int64_t* root_desc) {
+    if (root_desc) *root_desc = sizeof(to_root_desc);
+    if (perm_desc) *perm_desc = sizeof(to_perm_desc);
+    if (gemm_desc) *gemm_desc = sizeof(to_gemm_desc);
     if (join_desc) *join_desc = sizeof(to_join_desc);
     if (leaf_desc) *leaf_desc = sizeof(to_leaf_desc);
     if (scale_rec) *scale_rec = sizeof(to_scale_rec);
@@ -794,6 +1100,14 @@ int to_permute(const void* in, void* out, int rank, const int32_t* perm, int ele
                            batch);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? TO_OK : hip_fail(e, "to_permute launch");
+}
+
+int to_wall_clock_khz(int* khz) {
+    if (!khz) return TO_ERR_INVALID;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(khz, hipDeviceAttributeWallClockRate, dev);
+    return e == hipSuccess ? TO_OK : hip_fail(e, "hipDeviceGetAttribute(WallClockRate)");
 }
 
 int to_synchronize(void* stream) {

@@ -146,6 +146,56 @@ class Executable:
         self._pro_graph = self._wave_graph = None
 
 
+class BatchExecutable:
+    """A batch of unsliced plans resident on the device; ``run()`` contracts all of them
+    (one HIP graph) and returns their Results and per-instance completion times (s from
+    the batch start, device clock)."""
+
+    def __init__(self, api, prog):
+        self.api = api
+        self.prog = prog
+        self._graph = None
+        if api._use_graph:
+            arr = _lib.launch_array(prog.launches)
+            graph = ctypes.c_void_p()
+            _lib.check(api._lib.to_graph_create(ctypes.addressof(arr), len(prog.launches), api._entry_type,
+                                                ctypes.c_void_p(api._stream.cuda_stream), ctypes.byref(graph)),
+                       "to_graph_create")
+            self._graph = graph
+        khz = ctypes.c_int(0)
+        _lib.check(api._lib.to_wall_clock_khz(ctypes.byref(khz)), "to_wall_clock_khz")
+        self.clock_hz = khz.value * 1e3
+
+    def enqueue(self):
+        if self._graph is not None:
+            _lib.check(self.api._lib.to_graph_launch(self._graph, ctypes.c_void_p(self.api._stream.cuda_stream)),
+                       "to_graph_launch")
+        else:
+            self.api._run(self.prog.launches)
+
+    def results(self):
+        api = self.api
+        api.synchronize()
+        n = self.prog.count
+        size = ctypes.sizeof(_lib.Accum)
+        with api._torch.cuda.stream(api._stream):
+            raw = self.prog.accum[: n * size].cpu().numpy().tobytes()
+            start = int(self.prog.start[:8].cpu().numpy().view(np.int64)[0])
+        accs = [_lib.Accum.from_buffer_copy(raw[i * size:(i + 1) * size]) for i in range(n)]
+        res = [Result(a.mant, a.exp, a.count) for a in accs]
+        lat = [(a.pad - start) / self.clock_hz for a in accs]
+        return res, lat
+
+    def run(self):
+        self.enqueue()
+        return self.results()
+
+    def close(self):
+        if self._graph is not None:
+            self.api._lib.to_graph_destroy(self._graph)
+            self._graph = None
+
+
 class HipAPI(BaseTensorAPI):
     def __init__(self, device=0):
         import torch
@@ -161,6 +211,7 @@ class HipAPI(BaseTensorAPI):
         self._max_batch = 1 << 16
         self._mem_limit = None
         self._use_graph = True
+        self._regroup = True
         self.last_result = None
         self.last_program = None
         self.last_stats = {}
@@ -181,6 +232,8 @@ class HipAPI(BaseTensorAPI):
             self._mem_limit = None if value is None else int(value)
         elif key == "graph":
             self._use_graph = bool(value)
+        elif key == "regroup":
+            self._regroup = bool(value)
         else:
             super().add_argument(key, value)
 
@@ -244,7 +297,7 @@ class HipAPI(BaseTensorAPI):
 
     def _compile(self, tree, tables, num_groups, total_slices):
         """Compile and upload a plan; returns an ``Executable``."""
-        nodes = schedule.analyse(tree, tables)
+        nodes = schedule.analyse(tree, tables, self._regroup)
         budget = self._budget()
         fixed = sum(t.rows.size for t in tables.values()) * self._esize
         W = max(1, min(self._max_batch, total_slices))
@@ -255,7 +308,8 @@ class HipAPI(BaseTensorAPI):
             if W == 1:
                 raise OutOfMemoryError
             W = max(1, W // 2)
-        prog = schedule.compile_program(tree, tables, num_groups, self._entry_type, W, self._alloc)
+        prog = schedule.compile_program(tree, tables, num_groups, self._entry_type, W, self._alloc,
+                                        regroup=self._regroup)
         self._upload(prog.table, prog.table_host.tobytes())
         self._upload(prog.coef, prog.coef_host.tobytes())
         self._upload(prog.blob, prog.blob_host.tobytes())
@@ -294,6 +348,32 @@ class HipAPI(BaseTensorAPI):
             reordered = np.asarray(reordered, dtype=np.float64)
             tables[t] = schedule.LeafTable(reordered.reshape(reordered.shape[0], -1), lookup, len(shape))
         return total, tree, tables, len(groups)
+
+    def prepare_batch(self, plans):
+        """Compile many unsliced plans into one grouped schedule (``batch.compile_batch``)."""
+        from . import batch
+
+        instances = []
+        for plan in plans:
+            total, tree, tables, groups = self._plan_inputs(plan, None)
+            if groups:
+                raise ValueError("prepare_batch takes unsliced plans")
+            instances.append((tree, tables))
+        prog = batch.compile_batch(instances, self._entry_type, self._alloc, regroup=self._regroup)
+        self._upload(prog.table, prog.table_host.tobytes())
+        self._upload(prog.blob, prog.blob_host.tobytes())
+        if self._entry_type == _lib.TO_F32S:
+            self._upload(prog.recs, batch.rec_init_bytes(prog))
+        return BatchExecutable(self, prog)
+
+    def contract_batch(self, plans):
+        """Counts (floats) of many unsliced plans contracted together."""
+        exe = self.prepare_batch(plans)
+        try:
+            res, _ = exe.run()
+        finally:
+            exe.close()
+        return [r.value for r in res]
 
     def contract_sliced(self, execution_plan, num_slice_limit=None):
         total, tree, tables, groups = self._plan_inputs(execution_plan, num_slice_limit)
@@ -370,7 +450,39 @@ class HipAPI(BaseTensorAPI):
         if scaled:
             launches.append(_lib.Launch(op=_lib.TO_OP_SCALE_CLEAR, grid=1, aux0=crec.data_ptr(), iaux0=1))
         keep = [cbuf, crec]
-        if v["kind"] == "stream":
+        if v["kind"] == "gemm":
+            ops = []
+            for need, T, fpos, spos in ((v["perm_x"], X, g["xf"], g["xs"]), (v["perm_y"], Y, g["yf"], g["ys"])):
+                if not need:
+                    ops.append(T.data.data_ptr())
+                    continue
+                out_pos = [0] * T.rank
+                for t, p in enumerate(spos):
+                    out_pos[p] = t
+                for i, p in enumerate(fpos):
+                    out_pos[p] = len(spos) + i
+                dst = self._alloc((1 << T.rank) * self._esize)
+                pd, grid, lds = schedule.perm_desc(T.data.data_ptr(), dst.data_ptr(), T.rank, out_pos, 1,
+                                                   1 << T.rank, self._esize)
+                pblob = self._alloc(ctypes.sizeof(pd))
+                self._upload(pblob, bytes(pd))
+                launches.append(_lib.Launch(op=_lib.TO_OP_PERMUTE, count=1, grid=grid, lds=lds,
+                                            descs=pblob.data_ptr()))
+                keep += [dst, pblob]
+                ops.append(dst.data_ptr())
+            gd = _lib.GemmDesc()
+            gd.a, gd.b, gd.c = ops[0], ops[1], cbuf.data_ptr()
+            gd.ldcm, gd.ldcn = 1 << g["shf"], 1 << g["shy"]
+            if scaled:
+                gd.as_, gd.bs, gd.cs = X.rec.data_ptr(), Y.rec.data_ptr(), crec.data_ptr()
+            gd.mlog, gd.nlog, gd.klog = g["fx"], g["fy"], g["s"]
+            gd.nbatch, gd.block0, gd.nblocks = 1, 0, v["nblocks"]
+            gblob = self._alloc(ctypes.sizeof(gd))
+            self._upload(gblob, bytes(gd))
+            launches.append(_lib.Launch(op=_lib.TO_OP_GEMM, variant=v["tile"], count=1, grid=v["nblocks"],
+                                        descs=gblob.data_ptr()))
+            keep.append(gblob)
+        elif v["kind"] == "stream":
             d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
             blob = self._alloc(ctypes.sizeof(d))
             self._upload(blob, bytes(d))

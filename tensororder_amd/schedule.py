@@ -99,6 +99,92 @@ def flatten_tree(tree):
     return nodes
 
 
+def _join_props(lfree, rfree):
+    """(left_edge_map, right_edge_map, free edges) of a join (contraction_tree.pyx:248-288)."""
+    rpos = {e: i for i, e in enumerate(rfree)}
+    lset = set(lfree)
+    lmap, rmap, free = [], [], []
+    for i, e in enumerate(lfree):
+        if e in rpos:
+            lmap.append(i)
+            rmap.append(rpos[e])
+        else:
+            free.append(e)
+    free += [e for e in rfree if e not in lset]
+    return lmap, rmap, free
+
+
+class _T:
+    """A mutable tree node used while regrouping."""
+
+    __slots__ = ("leaf", "tensor", "free", "left", "right", "batched")
+
+    def __init__(self, leaf, tensor=None, free=None, left=None, right=None, batched=False):
+        self.leaf, self.tensor, self.free = leaf, tensor, free
+        self.left, self.right, self.batched = left, right, batched
+
+
+def _mk_join(left, right):
+    _, _, free = _join_props(left.free, right.free)
+    return _T(False, free=free, left=left, right=right, batched=left.batched or right.batched)
+
+
+def regroup_absorptions(nodes, small=8, max_shared=5, big=14):
+    """Reassociate chains in which a large tensor absorbs small operands one at a time.
+
+    ``((B . s1) . s2)`` becomes ``B . (s1 . s2)`` when s1 and s2 are small (rank <= small),
+    B is large (rank >= big) and the combined small operand keeps rank <= small with at most
+    ``max_shared`` bits shared with B.  The large tensor is then read and written once per
+    group instead of once per small operand; the value is the same contraction.  Returns a
+    new post-order node list (fresh ``Node`` objects with recomputed edge maps)."""
+    built = []
+    for n in nodes:
+        if n.is_leaf:
+            built.append(_T(True, tensor=n.tensor, free=list(n.free), batched=n.batched))
+            continue
+        t = _mk_join(built[n.left], built[n.right])
+        while True:
+            l, r = t.left, t.right
+            bigc, smallc = (l, r) if len(l.free) >= len(r.free) else (r, l)
+            if len(smallc.free) > small or len(bigc.free) < big or bigc.leaf:
+                break
+            a, b = bigc.left, bigc.right
+            inner_big, inner_small = (a, b) if len(a.free) >= len(b.free) else (b, a)
+            if len(inner_small.free) > small or len(inner_big.free) < big:
+                break
+            m = _mk_join(inner_small, smallc)
+            shared = len(set(m.free) & set(inner_big.free))
+            if len(m.free) > small or shared > max_shared:
+                break
+            t = _mk_join(inner_big, m)
+        built.append(t)
+    out, index = [], {}
+    stack = [(built[-1], False)]
+    while stack:
+        t, expanded = stack.pop()
+        if t.leaf or expanded:
+            n = Node(len(out))
+            n.is_leaf = t.leaf
+            n.free = list(t.free)
+            n.rank = len(t.free)
+            n.batched = t.batched
+            if t.leaf:
+                n.tensor = t.tensor
+            else:
+                n.left = index[id(t.left)]
+                n.right = index[id(t.right)]
+                n.lmap, n.rmap, _ = _join_props(out[n.left].free, out[n.right].free)
+                out[n.left].parent = n.idx
+                out[n.right].parent = n.idx
+            index[id(t)] = n.idx
+            out.append(n)
+        else:
+            stack.append((t, True))
+            stack.append((t.right, False))
+            stack.append((t.left, False))
+    return out
+
+
 def join_geometry(lrank, rrank, lmap, rmap):
     """Bit placement of one join in numpy tensordot order.
 
@@ -121,10 +207,28 @@ def join_geometry(lrank, rrank, lmap, rmap):
                 xf=b_free, xs=b_sh, yf=a_free, ys=a_sh)
 
 
+GEMM_TILES = 128  # largest GEMM output tile edge to use (64 or 128)
+GEMM_MIN_FREE = 6
+GEMM_MIN_SHARED = 4
+MAX_GEMM_BLOCKS = 8192
+
+
+def _canonical(free_pos, shared_pos):
+    """True when an operand is already row-major [free][shared] (shared bits lowest)."""
+    s = len(shared_pos)
+    return list(shared_pos) == list(range(s)) and list(free_pos) == list(range(s, s + len(free_pos)))
+
+
 def choose_variant(g, nb, esize):
     """Kernel variant and launch shape for a join with ``nb`` batch items."""
     fx, fy, s = g["fx"], g["fy"], g["s"]
     K = 1 << s
+    if fx >= GEMM_MIN_FREE and fy >= GEMM_MIN_FREE and s >= GEMM_MIN_SHARED:
+        big = GEMM_TILES == 128 and fx >= 7 and fy >= 7 and s >= (4 if esize == 8 else 5)
+        tb = 7 if big else 6
+        tiles = nb << (fx - tb + fy - tb)
+        return dict(kind="gemm", tile=1 if big else 0, nblocks=int(min(tiles, MAX_GEMM_BLOCKS)),
+                    perm_x=not _canonical(g["xf"], g["xs"]), perm_y=not _canonical(g["yf"], g["ys"]))
     yc = min(16, 1 << fy)
     if fx >= 8:
         while yc > 1 and 8 * K + yc * K * esize > LDS_LIMIT:
@@ -178,6 +282,48 @@ class Program:
         self.stats = {}
 
 
+def perm_plan(rank, out_pos, max_tile_bits=10):
+    """LDS tile of a bit permutation (input bit p -> output bit out_pos[p]): the union U of
+    the m lowest input bits and the input bits landing on the m lowest output bits, m as
+    large as keeps |U| <= max_tile_bits."""
+    ipos_of_out = [0] * rank
+    for p, q in enumerate(out_pos):
+        ipos_of_out[q] = p
+    U = set()
+    m = 0
+    while m < rank:
+        cand = U | {m, ipos_of_out[m]}
+        if len(cand) > max_tile_bits:
+            break
+        U = cand
+        m += 1
+    u_in = sorted(U)
+    v_out, v2u = [], []
+    for q in range(rank):
+        p = ipos_of_out[q]
+        if p in U:
+            v_out.append(q)
+            v2u.append(u_in.index(p))
+    rest = [p for p in range(rank) if p not in U]
+    return u_in, v_out, v2u, rest, [out_pos[p] for p in rest]
+
+
+def perm_desc(src, dst, rank, out_pos, nbatch, stride, esize):
+    """(to_perm_desc, grid, LDS bytes) of a batched bit permutation."""
+    u_in, v_out, v2u, rest_in, rest_out = perm_plan(rank, out_pos)
+    d = _lib.PermDesc()
+    d.in_, d.out = src, dst
+    d.inb = d.outb = stride
+    d.rank, d.nu, d.nrest, d.nbatch = rank, len(u_in), len(rest_in), nbatch
+    for arr, vals in ((d.u_in, u_in), (d.v_out, v_out), (d.v2u, v2u), (d.rest_in, rest_in), (d.rest_out, rest_out)):
+        for i, x in enumerate(vals):
+            arr[i] = x
+    ntiles = nbatch << len(rest_in)
+    d.block0 = 0
+    d.nblocks = int(min(ntiles, 8192))
+    return d, d.nblocks, esize << len(u_in)
+
+
 def _pos_array(vals):
     arr = _lib.c_i8x32()
     for i, v in enumerate(vals):
@@ -185,8 +331,27 @@ def _pos_array(vals):
     return arr
 
 
+def _temporaries(arena, n, nodes, W, esize, partial):
+    """Scratch buffers a join needs while it runs (split-K partial sums, permuted GEMM
+    operands); allocated after the output, released before the next join."""
+    v = n.geom["var"]
+    blocks = []
+    if v["kind"] == "splitk":
+        blocks.append(("partial", arena.alloc(v["nout"] * v["nsplit"] * esize)))
+    elif v["kind"] == "gemm":
+        g = n.geom
+        X, Y = (nodes[n.left], nodes[n.right]) if g["x_is_left"] else (nodes[n.right], nodes[n.left])
+        for key, op, need in (("perm_x", X, v["perm_x"]), ("perm_y", Y, v["perm_y"])):
+            if need:
+                nb = W if (n.batched and op.batched) else 1
+                blocks.append((key, arena.alloc(nb * (1 << op.rank) * esize)))
+    partial[n.idx] = dict(blocks)
+    for _, blk in blocks:
+        arena.release(blk)
+
+
 def plan_memory(nodes, tables, W, esize):
-    """Arena placement for batch W.  Returns (arena peak bytes, partial placements)."""
+    """Arena placement for batch W.  Returns (arena peak bytes, scratch placements)."""
     arena = _Arena()
     partial = {}
     # prologue: constant joins in post-order
@@ -194,9 +359,7 @@ def plan_memory(nodes, tables, W, esize):
         if n.is_leaf or n.batched:
             continue
         n.buf = arena.alloc((1 << n.rank) * esize)
-        if n.geom["var"]["kind"] == "splitk":
-            partial[n.idx] = arena.alloc(n.geom["var"]["nout"] * n.geom["var"]["nsplit"] * esize)
-            arena.release(partial[n.idx])
+        _temporaries(arena, n, nodes, 1, esize, partial)
         for c in (n.left, n.right):
             ch = nodes[c]
             if not ch.is_leaf:
@@ -209,9 +372,7 @@ def plan_memory(nodes, tables, W, esize):
         if n.is_leaf or not n.batched:
             continue
         n.buf = arena.alloc(W * (1 << n.rank) * esize)
-        if n.geom["var"]["kind"] == "splitk":
-            partial[n.idx] = arena.alloc(n.geom["var"]["nout"] * n.geom["var"]["nsplit"] * esize)
-            arena.release(partial[n.idx])
+        _temporaries(arena, n, nodes, W, esize, partial)
         for c in (n.left, n.right):
             ch = nodes[c]
             if ch.batched:
@@ -219,8 +380,9 @@ def plan_memory(nodes, tables, W, esize):
     return arena.peak, partial
 
 
-def analyse(tree, tables):
-    """Node list with batched flags (depends on a sliced leaf)."""
+def analyse(tree, tables, regroup=True):
+    """Node list with batched flags (depends on a sliced leaf); with ``regroup`` the
+    small-operand absorption chains are reassociated (``regroup_absorptions``)."""
     nodes = flatten_tree(tree)
     for n in nodes:
         if n.is_leaf:
@@ -232,6 +394,8 @@ def analyse(tree, tables):
             n.batched = nodes[n.left].batched or nodes[n.right].batched
     if nodes[-1].rank != 0:
         raise ValueError("the tree's root has free edges; only full contractions are supported")
+    if regroup:
+        nodes = regroup_absorptions(nodes)
     return nodes
 
 
@@ -247,7 +411,75 @@ def peak_bytes(nodes, tables, W, entry_type):
     return peak
 
 
-def compile_program(tree, tables, num_groups, entry_type, W, device_alloc):
+def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
+    """Descriptor records of one join, in execution order:
+    [(kind, op, variant, lds, descriptor struct, blocks)].  Descriptors carry block0 = 0;
+    a grouped launch rebases them."""
+    g, v = n.geom, n.geom["var"]
+    nb = W if n.batched else 1
+    X, Y = (nodes[n.left], nodes[n.right]) if g["x_is_left"] else (nodes[n.right], nodes[n.left])
+    out = []
+    if v["kind"] == "gemm":
+        operands = []
+        for key, op, fpos, spos in (("perm_x", X, g["xf"], g["xs"]), ("perm_y", Y, g["yf"], g["ys"])):
+            nb_op = W if (n.batched and op.batched) else 1
+            stride = (1 << op.rank) if (n.batched and op.batched) else 0
+            if v[key]:
+                dst = abase + scratch[key][0]
+                out_pos = [0] * op.rank
+                for t, p in enumerate(spos):
+                    out_pos[p] = t
+                for i, p in enumerate(fpos):
+                    out_pos[p] = len(spos) + i
+                pd, grid, lds = perm_desc(op.ptr, dst, op.rank, out_pos, nb_op, 1 << op.rank, esize)
+                out.append(("permute", _lib.TO_OP_PERMUTE, 0, lds, pd, grid))
+                operands.append((dst, stride, op))
+            else:
+                operands.append((op.ptr, stride, op))
+        (a, ab, xo), (b, bb, yo) = operands
+        gd = _lib.GemmDesc()
+        gd.a, gd.b, gd.c = a, b, n.ptr
+        gd.ab, gd.bb, gd.cb = ab, bb, n.bstride
+        gd.ldcm, gd.ldcn = 1 << g["shf"], 1 << g["shy"]
+        if scaled:
+            gd.as_, gd.bs, gd.cs = recp(xo), recp(yo), recp(n)
+            gd.asb = xo.rec_stride if n.batched else 0
+            gd.bsb = yo.rec_stride if n.batched else 0
+        gd.mlog, gd.nlog, gd.klog = g["fx"], g["fy"], g["s"]
+        gd.nbatch = nb
+        gd.block0, gd.nblocks = 0, v["nblocks"]
+        out.append(("gemm", _lib.TO_OP_GEMM, v["tile"], 0, gd, v["nblocks"]))
+        return out
+    d = JoinDesc()
+    d.x, d.y, d.c = X.ptr, Y.ptr, n.ptr
+    d.xb = X.bstride if n.batched else 0
+    d.yb = Y.bstride if n.batched else 0
+    d.cb = n.bstride
+    if scaled:
+        d.xs, d.ys, d.cs = recp(X), recp(Y), recp(n)
+        d.xsb = X.rec_stride if n.batched else 0
+        d.ysb = Y.rec_stride if n.batched else 0
+    d.fx, d.fy, d.s = g["fx"], g["fy"], g["s"]
+    d.shf, d.shy = g["shf"], g["shy"]
+    d.nbatch = nb
+    d.block0 = 0
+    d.xf_pos, d.xs_pos = _pos_array(g["xf"]), _pos_array(g["xs"])
+    d.yf_pos, d.ys_pos = _pos_array(g["yf"]), _pos_array(g["ys"])
+    if v["kind"] == "stream":
+        d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
+        out.append(("stream", _lib.TO_OP_JOIN_STREAM, v["yc"], v["lds"], d, v["nblocks"]))
+        return out
+    d.partial = abase + scratch["partial"][0]
+    d.nsplit = v["nsplit"]
+    d.nblocks = v["nblocks"]
+    out.append(("splitk", _lib.TO_OP_JOIN_SPLITK, 0, 0, d, v["nblocks"]))
+    d2 = JoinDesc.from_buffer_copy(bytes(d))
+    d2.nblocks = v["fin_blocks"]
+    out.append(("finish", _lib.TO_OP_JOIN_FINISH, 0, 0, d2, v["fin_blocks"]))
+    return out
+
+
+def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regroup=True):
     """Build a Program for batch size W.
 
     ``device_alloc(nbytes)`` returns an object with ``.data_ptr()`` (device memory that
@@ -255,7 +487,7 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc):
     esize = 8 if entry_type == _lib.TO_F64 else 4
     scaled = entry_type == _lib.TO_F32S
     fdtype = np.float64 if esize == 8 else np.float32
-    nodes = analyse(tree, tables)
+    nodes = analyse(tree, tables, regroup)
     prog = Program()
     prog.nbatch = W
     prog.nodes = nodes
@@ -354,39 +586,10 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc):
     pending = []  # (list, Launch, blob offset attr)
 
     def join_launches(n, phase_list):
-        g, v = n.geom, n.geom["var"]
-        nb = W if n.batched else 1
-        X, Y = (nodes[n.left], nodes[n.right]) if g["x_is_left"] else (nodes[n.right], nodes[n.left])
-        d = JoinDesc()
-        d.x, d.y, d.c = X.ptr, Y.ptr, n.ptr
-        d.xb = X.bstride if n.batched else 0
-        d.yb = Y.bstride if n.batched else 0
-        d.cb = n.bstride
-        if scaled:
-            d.xs, d.ys, d.cs = recp(X), recp(Y), recp(n)
-            d.xsb = X.rec_stride if n.batched else 0
-            d.ysb = Y.rec_stride if n.batched else 0
-        d.fx, d.fy, d.s = g["fx"], g["fy"], g["s"]
-        d.shf, d.shy = g["shf"], g["shy"]
-        d.nbatch = nb
-        d.block0 = 0
-        d.xf_pos, d.xs_pos = _pos_array(g["xf"]), _pos_array(g["xs"])
-        d.yf_pos, d.ys_pos = _pos_array(g["yf"]), _pos_array(g["ys"])
-        if v["kind"] == "stream":
-            d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
-            off = put([d])
-            phase_list.append(("stream", Launch(op=_lib.TO_OP_JOIN_STREAM, variant=v["yc"], count=1,
-                                                grid=v["nblocks"], lds=v["lds"]), off, n.idx))
-        else:
-            d.partial = abase + partial[n.idx][0]
-            d.nsplit = v["nsplit"]
-            d.nblocks = v["nblocks"]
-            off = put([d])
-            phase_list.append(("splitk", Launch(op=_lib.TO_OP_JOIN_SPLITK, count=1, grid=v["nblocks"]), off, n.idx))
-            d2 = JoinDesc.from_buffer_copy(bytes(d))
-            d2.nblocks = v["fin_blocks"]
-            off2 = put([d2])
-            phase_list.append(("finish", Launch(op=_lib.TO_OP_JOIN_FINISH, count=1, grid=v["fin_blocks"]), off2, n.idx))
+        for kind, op, variant, lds, st, nblk in join_records(n, nodes, W, esize, scaled, abase,
+                                                               partial.get(n.idx, {}), recp):
+            off = put([st])
+            phase_list.append((kind, Launch(op=op, variant=variant, count=1, grid=nblk, lds=lds), off, n.idx))
 
     pro, wave = [], []
     if scaled and prog.rec_ranges[1] > prog.rec_ranges[0]:
@@ -451,6 +654,7 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc):
         joins=sum(1 for n in nodes if not n.is_leaf),
         batched_joins=sum(1 for n in nodes if not n.is_leaf and n.batched),
         splitk=sum(1 for n in nodes if not n.is_leaf and n.geom["var"]["kind"] == "splitk"),
+        gemm=sum(1 for n in nodes if not n.is_leaf and n.geom["var"]["kind"] == "gemm"),
         arena_bytes=peak,
         table_bytes=int(table_host.nbytes),
         launches_prologue=len(prog.prologue),

@@ -67,6 +67,7 @@ def _scale_exp(bits):
 
 class Emulator:
     def __init__(self, heap, entry_type):
+        self.clock = 0
         self.heap = heap
         self.scaled = entry_type == _lib.TO_F32S
         self.dt = np.float32 if self.scaled else np.float64
@@ -173,17 +174,78 @@ class Emulator:
         off = L.aux2 - HEAP_BASE
         self.heap.mem[off:off + ctypes.sizeof(_lib.Accum)] = np.frombuffer(bytes(acc), np.uint8)
 
+    def op_9(self, L):
+        for i in range(L.count):
+            d = self.heap.read_struct(_lib.PermDesc, L.descs + i * ctypes.sizeof(_lib.PermDesc))
+            n = 1 << d.rank
+            out_pos = {}
+            for j in range(d.nu):
+                out_pos[d.u_in[j]] = d.v_out[[k for k in range(d.nu) if d.v2u[k] == j][0]]
+            for j in range(d.nrest):
+                out_pos[d.rest_in[j]] = d.rest_out[j]
+            idx = np.arange(n, dtype=np.int64)
+            dst_idx = np.zeros(n, dtype=np.int64)
+            for p, q in out_pos.items():
+                dst_idx |= ((idx >> p) & 1) << q
+            esz = self.dt().itemsize
+            for b in range(d.nbatch):
+                src = self.heap.view(d.in_ + b * d.inb * esz, self.dt, n)
+                dst = self.heap.view(d.out + b * d.outb * esz, self.dt, n)
+                dst[dst_idx] = src
+
+    def op_10(self, L):
+        for i in range(L.count):
+            d = self.heap.read_struct(_lib.GemmDesc, L.descs + i * ctypes.sizeof(_lib.GemmDesc))
+            M, N, K = 1 << d.mlog, 1 << d.nlog, 1 << d.klog
+            esz = self.dt().itemsize
+            m = np.arange(M, dtype=np.int64)
+            nn = np.arange(N, dtype=np.int64)
+            cidx = m[:, None] * d.ldcm + nn[None, :] * d.ldcn
+            for b in range(d.nbatch):
+                A = self.heap.view(d.a + b * d.ab * esz, self.dt, M * K).reshape(M, K).astype(np.float64)
+                B = self.heap.view(d.b + b * d.bb * esz, self.dt, N * K).reshape(N, K).astype(np.float64)
+                ea = eb = 0
+                if self.scaled:
+                    e0, m0 = self.get_rec(d.as_, b * d.asb)
+                    e1, m1 = self.get_rec(d.bs, b * d.bsb)
+                    A = A * 2.0 ** -_scale_exp(m0)
+                    B = B * 2.0 ** -_scale_exp(m1)
+                    ea, eb = e0 + _scale_exp(m0), e1 + _scale_exp(m1)
+                Cm = (A @ B.T).astype(self.dt)
+                C = self.heap.view(d.c + b * d.cb * esz, self.dt, int(cidx.max() + 1))
+                C[cidx] = Cm
+                if self.scaled:
+                    mx = float(np.max(np.abs(Cm)))
+                    self.set_rec(d.cs, b, exp=ea + eb, maxbits=int(np.array([mx], np.float32).view(np.uint32)[0]))
+
+    def op_11(self, L):
+        for i in range(L.count):
+            d = self.heap.read_struct(_lib.RootDesc, L.descs + i * ctypes.sizeof(_lib.RootDesc))
+            v = float(self.heap.view(d.root, self.dt, 1)[0])
+            if self.scaled:
+                v = v * 2.0 ** self.get_rec(d.rec, 0)[0]
+            m, e = math.frexp(v)
+            acc = _lib.Accum(mant=m, exp=e if m else 0, count=1, pad=self.clock)
+            self.clock += 1
+            off = d.acc - HEAP_BASE
+            self.heap.mem[off:off + ctypes.sizeof(_lib.Accum)] = np.frombuffer(bytes(acc), np.uint8)
+
+    def op_12(self, L):
+        raw = self.heap.view(L.aux2, np.uint8, L.iaux0 * ctypes.sizeof(_lib.Accum))
+        raw[:] = 0
+        self.clock = 1
+
     def op_7(self, L):
         params = self.heap.view(L.aux3, np.int64, 2)
         params[0] += L.iaux0
 
 
-def emulate_plan(tree, tables, num_groups, total, entry_type, W):
+def emulate_plan(tree, tables, num_groups, total, entry_type, W, regroup=True):
     """Compile and emulate; returns (value, program)."""
     from tensororder_amd import schedule
 
     heap = Heap()
-    prog = schedule.compile_program(tree, tables, num_groups, entry_type, W, heap.alloc)
+    prog = schedule.compile_program(tree, tables, num_groups, entry_type, W, heap.alloc, regroup=regroup)
     heap.write(prog.table, prog.table_host.tobytes())
     heap.write(prog.coef, prog.coef_host.tobytes())
     heap.write(prog.blob, prog.blob_host.tobytes())
@@ -231,3 +293,22 @@ class HostOps:
     @staticmethod
     def transpose(t, perm):
         return np.transpose(t, axes=perm)
+
+
+def emulate_batch(instances, entry_type):
+    """Compile a batch of (tree, tables) and emulate it; returns the per-instance values."""
+    from tensororder_amd import batch
+
+    heap = Heap(1 << 29)
+    prog = batch.compile_batch(instances, entry_type, heap.alloc)
+    heap.write(prog.table, prog.table_host.tobytes())
+    heap.write(prog.blob, prog.blob_host.tobytes())
+    if entry_type == _lib.TO_F32S:
+        heap.write(prog.recs, batch.rec_init_bytes(prog))
+    emu = Emulator(heap, entry_type)
+    emu.run(prog.launches)
+    out = []
+    for i in range(prog.count):
+        acc = heap.read_struct(_lib.Accum, prog.accum.data_ptr() + i * ctypes.sizeof(_lib.Accum))
+        out.append(math.ldexp(acc.mant, acc.exp))
+    return out, prog

@@ -58,7 +58,11 @@ def _two_tensor_plan(ra, rb, shared, seed):
     return SlicedExecutionPlan(ctx.get_tree(root), net)
 
 
-@pytest.mark.parametrize("ra,rb,s", [(12, 12, 12), (13, 11, 10), (9, 14, 9), (6, 6, 0), (10, 3, 1)])
+DENSE = [(12, 12, 12), (13, 11, 10), (9, 14, 9), (6, 6, 0), (10, 3, 1), (12, 12, 6), (14, 11, 5), (11, 15, 5),
+         (10, 10, 4)]
+
+
+@pytest.mark.parametrize("ra,rb,s", DENSE)
 @pytest.mark.parametrize("et,tol", [(_lib.TO_F64, 1e-11), (_lib.TO_F32S, 2e-5)])
 def test_dense_joins_emulated(ra, rb, s, et, tol):
     plan = _two_tensor_plan(ra, rb, s, seed=ra * 100 + rb)
@@ -68,6 +72,8 @@ def test_dense_joins_emulated(ra, rb, s, et, tol):
     got, prog = emulator.emulate_plan(tree, tables, G, total, et, 1)
     if s >= 9 and ra + rb - 2 * s <= 6:
         assert prog.stats["splitk"] >= 1
+    if ra - s >= 6 and rb - s >= 6 and s >= 4:
+        assert prog.stats["gemm"] >= 1
     assert got == pytest.approx(want, rel=tol, abs=tol * 1e-3 * abs(want))
 
 
@@ -132,3 +138,22 @@ def test_memory_plan_never_overlaps_live_buffers():
                     s1, z1 = m.buf
                     assert s0 + z0 <= s1 or s1 + z1 <= s0, (n.idx, m.idx)
             done.add(n.idx)
+
+
+@pytest.mark.parametrize("et,tol", [(_lib.TO_F64, 1e-12), (_lib.TO_F32S, 1e-6)])
+def test_batch_of_uneven_instances_emulated(et, tol):
+    from tensororder_amd import network, planners
+    from tensororder_amd.formula import parse_dimacs_text
+    from tensororder_amd.instances import vertex_cover_cnf
+
+    instances, wants = [], []
+    for seed, n in enumerate([10, 16, 24, 12, 30, 20]):
+        f = parse_dimacs_text(vertex_cover_cnf(n, seed, True), "cachet")
+        plan = planners.best_plan(network.cnf_count(f), "line", solver=None)
+        wants.append(numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan)))
+        tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
+        instances.append((tree, tables))
+    got, prog = emulator.emulate_batch(instances, et)
+    assert prog.stats["steps"] < prog.stats["joins"]
+    for g, w in zip(got, wants):
+        assert g == pytest.approx(w, rel=tol)

@@ -111,6 +111,32 @@ def test_graph_and_eager_agree(hip):
     assert g == pytest.approx(_expected(doc), rel=F64_TOL)
 
 
+@pytest.mark.parametrize("ra,rb,s", [(12, 12, 6), (14, 11, 5), (11, 15, 5), (10, 10, 4), (13, 11, 10),
+                                     (12, 12, 12)])
+def test_dense_join_plans(hip, hip32, ra, rb, s):
+    from test_emulator import _two_tensor_plan
+
+    _say("case dense", ra, rb, s)
+    plan = _two_tensor_plan(ra, rb, s, seed=ra * 100 + rb)
+    want = numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan))
+    got = hip.contract_sliced(plan)
+    assert got == pytest.approx(want, rel=1e-11)
+    got32 = hip32.contract_sliced(plan)
+    assert got32 == pytest.approx(want, rel=2e-5)
+
+
+def test_tensordot_gemm_shapes(hip):
+    rng = np.random.default_rng(5)
+    _say("case tensordot gemm")
+    for ra, rb, s in ((13, 12, 6), (12, 14, 7), (10, 10, 4)):
+        a = rng.standard_normal((2,) * ra)
+        b = rng.standard_normal((2,) * rb)
+        la = [int(x) for x in rng.permutation(ra)[:s]]
+        lb = [int(x) for x in rng.permutation(rb)[:s]]
+        got = hip.tensordot(a, b, (la, lb)).numpy()
+        np.testing.assert_allclose(got, np.tensordot(a, b, (la, lb)), rtol=1e-11, atol=1e-11)
+
+
 def test_tensordot_matches_numpy(hip):
     rng = np.random.default_rng(1)
     _say("case tensordot")
@@ -161,3 +187,26 @@ def test_execution_run_slices_on_oom(hip):
         hip.add_argument("mem_limit", None)
     assert len(plan.groups_to_slice) > 0
     assert got == pytest.approx(_expected(doc), rel=F64_TOL)
+
+
+def test_batch_of_uneven_instances(hip, hip32):
+    from tensororder_amd import network, planners
+    from tensororder_amd.formula import parse_dimacs_text
+    from tensororder_amd.instances import mixed_batch_sizes, vertex_cover_cnf
+
+    _say("case batch of instances")
+    plans, wants = [], []
+    for seed, n in enumerate(mixed_batch_sizes(24, 7, low=20, high=60)):
+        f = parse_dimacs_text(vertex_cover_cnf(n, seed, True), "cachet")
+        plan = planners.best_plan(network.cnf_count(f), "line", solver=None)
+        plans.append(plan)
+        wants.append(numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan)))
+    for api, tol in ((hip, F64_TOL), (hip32, F32_TOL)):
+        exe = api.prepare_batch(plans)
+        try:
+            for rep in range(3):  # graph replays, each checked
+                res, lat = exe.run()
+                assert [r.value for r in res] == pytest.approx(wants, rel=tol)
+                assert all(l >= 0 for l in lat)
+        finally:
+            exe.close()
