@@ -16,7 +16,8 @@
  *         sum_k X[batch*xb + pdep(f, xf_pos) + pdep(k, xs_pos)]
  *             * Y[batch*yb + pdep(y, yf_pos) + pdep(k, ys_pos)]
  * where pdep(v, pos) places bit i of v at bit position pos[i].  A batch stride of 0
- * broadcasts an operand across the batch (slice-invariant tensors).
+ * broadcasts an operand across the batch (slice-invariant tensors).  Free-bit positions
+ * (xf_pos, yf_pos) must increase with the bit index.
  *
  * Entry types: TO_F64 computes in float64 throughout.  TO_F32S computes in float32 and
  * keeps, per tensor and batch item, a power-of-two exponent (to_scale_rec): true value =
@@ -55,8 +56,9 @@ extern "C" {
 #define TO_OP_WAVE_ADVANCE 7 /* aux3: wave params; adds iaux0 to params[0] */
 #define TO_OP_STEP_BEGIN 8   /* aux2: to_accum* (zeroed), aux3: wave params (params[0] = 0) */
 #define TO_OP_PERMUTE 9      /* descs: to_perm_desc[count] */
-#define TO_OP_GEMM 10        /* descs: to_gemm_desc[count]; variant 0: 64x64 output tiles
-                                (mlog, nlog >= 6), variant 1: 128x128 tiles (mlog, nlog >= 7) */
+#define TO_OP_GEMM 10        /* descs: to_gemm_desc[count]; variant bit 0: 128x128 output
+                                tiles (mlog, nlog >= 7) instead of 64x64 (>= 6); bit 1: float
+                                partial sums accumulated in double every 4 K-steps */
 #define TO_OP_ROOT_STORE 11  /* descs: to_root_desc[count]: one instance's root -> its accumulator */
 #define TO_OP_BATCH_BEGIN 12 /* aux2: to_accum[iaux0] zeroed; aux3: int64 start time (device clock) */
 
@@ -75,7 +77,11 @@ typedef struct to_join_desc {
     int32_t block0, nblocks;   /* blocks of this join within its launch */
     int32_t nych;              /* y chunks (stream) */
     int32_t nsplit;            /* k splits (split-K) */
-    int32_t pad0, pad1;
+    int32_t lnych;             /* log2(nych) */
+    int32_t stage;             /* stream, fx >= 8, bit 0: outputs staged through LDS (shy == 0,
+                                  shf == fy, one y chunk; 256 * 2^fy entries after the Y
+                                  slab); bit 1: X tile staged (all shared bits of X below
+                                  bit 8 + s; 256 * 2^s entries after that) */
     int8_t xf_pos[TO_MAX_BITS];
     int8_t xs_pos[TO_MAX_BITS];
     int8_t yf_pos[TO_MAX_BITS];

@@ -44,7 +44,7 @@ class JoinDesc(ctypes.Structure):
         ("nbatch", ctypes.c_int32),
         ("block0", ctypes.c_int32), ("nblocks", ctypes.c_int32),
         ("nych", ctypes.c_int32), ("nsplit", ctypes.c_int32),
-        ("pad0", ctypes.c_int32), ("pad1", ctypes.c_int32),
+        ("lnych", ctypes.c_int32), ("stage", ctypes.c_int32),
         ("xf_pos", c_i8x32), ("xs_pos", c_i8x32), ("yf_pos", c_i8x32), ("ys_pos", c_i8x32),
     ]
 
@@ -114,8 +114,10 @@ class Launch(ctypes.Structure):
 EXPORTED = [
     "to_version", "to_struct_sizes", "to_last_error", "to_device_count", "to_set_device",
     "to_run", "to_run_timed", "to_graph_create", "to_graph_launch", "to_graph_destroy", "to_permute",
-    "to_wall_clock_khz", "to_synchronize",
+    "to_wall_clock_khz", "to_synchronize", "to_mlp_forward", "to_mlp_backward", "to_mlp_adam",
 ]
+MLP_IN, MLP_H1, MLP_H2, MLP_OUT = 16, 64, 64, 2
+MLP_NPARAMS = 64 * 16 + 64 + 64 * 64 + 64 + 2 * 64 + 2
 
 _lib = None
 
@@ -149,6 +151,10 @@ def load(path=LIB_PATH):
     lib.to_graph_destroy.argtypes = [vp]
     lib.to_permute.argtypes = [vp, vp, i32, vp, i32, i64, vp]
     lib.to_synchronize.argtypes = [vp]
+    lib.to_mlp_forward.argtypes = [vp, vp, i64, vp, vp, vp, vp]
+    lib.to_mlp_backward.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp]
+    lib.to_mlp_adam.argtypes = [vp, vp, vp, vp, i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_float, i64, vp]
     if lib.to_version() != ABI_VERSION:
         raise ImportError("libtensororder_hip.so ABI version mismatch; rebuild it")
     sizes = [ctypes.c_int64() for _ in range(8)]

@@ -162,7 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
     const int tid = threadIdx.x;
     const int fx = d->fx, fy = d->fy, s = d->s;
     const int K = 1 << s;
-    const int nych = d->nych;
+    const int lnych = d->lnych;
+    const int nych = 1 << lnych;
     const int nbatch = d->nbatch;
     const T* __restrict__ X = reinterpret_cast<const T*>(d->x);
     const T* __restrict__ Y = reinterpret_cast<const T*>(d->y);
@@ -184,18 +185,28 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
     const int nblk = d->nblocks;
 
     if (fx >= 8) {
-        // tiles of 256 consecutive f inside one batch item; tile order puts y chunks
-        // innermost so a block revisits the same X rows while they are in cache
-        const int64_t nft = 1ll << (fx - 8);
-        const int64_t ntiles = (int64_t)nbatch * nft * nych;
+        // tiles of 256 consecutive f inside one batch item, y chunks innermost so a block
+        // revisits the same X rows while they are in cache.  Tile t = ((batch * nft + ft)
+        // << lnych) | ych; the counters advance incrementally (no division per tile).
+        const int hb = fx - 8;
+        const int64_t nft = 1ll << hb;
+        const int64_t ntiles = (int64_t)nbatch << (hb + lnych);
         const int64_t t0 = ntiles * local / nblk, t1 = ntiles * (local + 1) / nblk;
         const int64_t xlo = pdep(tid, d->xf_pos, 8);
+        // free-bit positions increase with the bit index, so the offsets of consecutive ft
+        // follow x' = ((x | ~mask) + 1) & mask
+        const int64_t hmask = pdep(~0ull, d->xf_pos + 8, hb);
+        int ych = (int)(t0 & (nych - 1));
+        int64_t ft = (t0 >> lnych) & (nft - 1);
+        int64_t batch = t0 >> (lnych + hb);
+        int64_t xhi = pdep(ft, d->xf_pos + 8, hb);
         int64_t staged = -1;
+        // Y-free bits below the f bits, all of them in one chunk: stage the outputs
+        const bool stage_out = (d->stage & 1) != 0;
+        const bool stage_in = (d->stage & 2) != 0;
+        T* otile = ylds + YC * K;
+        T* xtile = otile + (stage_out ? (kBlock << fy) : 0);
         for (int64_t t = t0; t < t1; ++t) {
-            const int ych = (int)(t % nych);
-            const int64_t bf = t / nych;
-            const int64_t ft = bf % nft;
-            const int64_t batch = bf / nft;
             const int64_t key = batch * nych + ych;
             const int ny = min(YC, NY - ych * YC);
             float sx = 1.0f, sy = 1.0f;
@@ -218,14 +229,27 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                 staged = key;
             }
             const int64_t f = (ft << 8) | tid;
-            const T* Xp = X + batch * xb + pdep(ft, d->xf_pos + 8, fx - 8) + xlo;
+            const T* Xp = X + batch * xb + xhi + xlo;
+            if (stage_in) {
+                // the tile's X entries are the contiguous run of 2^(8+s) entries at the
+                // high-bit offset: load it linearly, then read this thread's entries in LDS
+                const T* src = X + batch * xb + xhi;
+                __syncthreads();
+                for (int i = tid; i < (kBlock << s); i += kBlock) xtile[i] = src[i];
+                __syncthreads();
+            }
             T acc[YC];
 #pragma unroll
             for (int yc = 0; yc < YC; ++yc) acc[yc] = T(0);
             for (int k0 = 0; k0 < K; k0 += 8) {
                 T xv[8];
+                if (stage_in) {
 #pragma unroll
-                for (int kk = 0; kk < 8; ++kk) xv[kk] = (k0 + kk < K) ? Xp[kx[k0 + kk]] * T(sx) : T(0);
+                    for (int kk = 0; kk < 8; ++kk) xv[kk] = (k0 + kk < K) ? xtile[xlo + kx[k0 + kk]] * T(sx) : T(0);
+                } else {
+#pragma unroll
+                    for (int kk = 0; kk < 8; ++kk) xv[kk] = (k0 + kk < K) ? Xp[kx[k0 + kk]] * T(sx) : T(0);
+                }
 #pragma unroll
                 for (int yc = 0; yc < YC; ++yc) {
 #pragma unroll
@@ -236,12 +260,28 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
             }
             T* Cb = C + batch * cb;
             float m = 0.0f;
+            if (SCALED) {
 #pragma unroll
-            for (int yc = 0; yc < YC; ++yc) {
-                if (yc < ny) {
-                    const int64_t y = (int64_t)ych * YC + yc;
-                    Cb[(f << shf) | (y << shy)] = acc[yc];
-                    if (SCALED) m = fmaxf(m, (float)absval(acc[yc]));
+                for (int yc = 0; yc < YC; ++yc)
+                    if (yc < ny) m = fmaxf(m, (float)absval(acc[yc]));
+            }
+            if (stage_out) {
+                // the tile's outputs are one contiguous run of 256 * 2^fy entries: stage
+                // them in LDS and store them with consecutive lanes on consecutive entries
+                __syncthreads();
+#pragma unroll
+                for (int yc = 0; yc < YC; ++yc)
+                    if (yc < NY) otile[tid * NY + yc] = acc[yc];
+                __syncthreads();
+                T* dst = Cb + (ft << (8 + fy));
+                for (int i = tid; i < (kBlock << fy); i += kBlock) dst[i] = otile[i];
+            } else {
+#pragma unroll
+                for (int yc = 0; yc < YC; ++yc) {
+                    if (yc < ny) {
+                        const int64_t y = (int64_t)ych * YC + yc;
+                        Cb[(f << shf) | (y << shy)] = acc[yc];
+                    }
                 }
             }
             if (SCALED) {
@@ -249,55 +289,70 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                 if ((tid & 63) == 0) note_max(d->cs, batch, m);
                 if (tid == 0 && ft == 0 && ych == 0) note_exp(d->cs, batch, ex + ey);
             }
+            if (++ych == nych) {
+                ych = 0;
+                if (++ft == nft) {
+                    ft = 0;
+                    xhi = 0;
+                    ++batch;
+                } else {
+                    xhi = ((xhi | ~hmask) + 1) & hmask;
+                }
+            }
         }
     } else {
         // tiles of 256 consecutive (batch item, f) pairs; Y read through the cache
         const int fmask = (1 << fx) - 1;
         const int64_t nL = (int64_t)nbatch << fx;
         const int64_t nLt = (nL + kBlock - 1) / kBlock;
-        const int64_t ntiles = nLt * nych;
+        const int64_t ntiles = nLt << lnych;
         const int64_t t0 = ntiles * local / nblk, t1 = ntiles * (local + 1) / nblk;
         const int64_t xoff = pdep(tid & fmask, d->xf_pos, fx);
         const int f = tid & fmask;
+        int ych = (int)(t0 & (nych - 1));
+        int64_t lt = t0 >> lnych;
         for (int64_t t = t0; t < t1; ++t) {
-            const int ych = (int)(t % nych);
-            const int64_t lt = t / nych;
             const int64_t L = lt * kBlock + tid;
-            if (L >= nL) continue;
-            const int64_t batch = L >> fx;
-            const int ny = min(YC, NY - ych * YC);
-            float sx = 1.0f, sy = 1.0f;
-            int64_t ex = 0, ey = 0;
-            if (SCALED) {
-                const OpScale a = op_scale(d->xs, d->xsb, batch);
-                const OpScale c = op_scale(d->ys, d->ysb, batch);
-                sx = a.f; ex = a.e; sy = c.f; ey = c.e;
-            }
-            const T* Xp = X + batch * xb + xoff;
-            const T* Yb = Y + batch * yb;
-            int64_t yoff[YC];
+            if (L < nL) {
+                const int64_t batch = L >> fx;
+                const int ny = min(YC, NY - ych * YC);
+                float sx = 1.0f, sy = 1.0f;
+                int64_t ex = 0, ey = 0;
+                if (SCALED) {
+                    const OpScale a = op_scale(d->xs, d->xsb, batch);
+                    const OpScale c = op_scale(d->ys, d->ysb, batch);
+                    sx = a.f; ex = a.e; sy = c.f; ey = c.e;
+                }
+                const T* Xp = X + batch * xb + xoff;
+                const T* Yb = Y + batch * yb;
+                int64_t yoff[YC];
 #pragma unroll
-            for (int yc = 0; yc < YC; ++yc) yoff[yc] = (yc < ny) ? pdep(ych * YC + yc, d->yf_pos, fy) : 0;
-            T acc[YC];
+                for (int yc = 0; yc < YC; ++yc) yoff[yc] = (yc < ny) ? pdep(ych * YC + yc, d->yf_pos, fy) : 0;
+                T acc[YC];
 #pragma unroll
-            for (int yc = 0; yc < YC; ++yc) acc[yc] = T(0);
-            for (int k = 0; k < K; ++k) {
-                const T xv = Xp[kx[k]] * T(sx);
+                for (int yc = 0; yc < YC; ++yc) acc[yc] = T(0);
+                for (int k = 0; k < K; ++k) {
+                    const T xv = Xp[kx[k]] * T(sx);
+#pragma unroll
+                    for (int yc = 0; yc < YC; ++yc) {
+                        if (yc < ny) acc[yc] += xv * (Yb[yoff[yc] + ky[k]] * T(sy));
+                    }
+                }
+                T* Cb = C + batch * cb;
 #pragma unroll
                 for (int yc = 0; yc < YC; ++yc) {
-                    if (yc < ny) acc[yc] += xv * (Yb[yoff[yc] + ky[k]] * T(sy));
+                    if (yc < ny) {
+                        const int64_t y = (int64_t)ych * YC + yc;
+                        Cb[((int64_t)f << shf) | (y << shy)] = acc[yc];
+                        if (SCALED) note_max(d->cs, batch, (float)absval(acc[yc]));
+                    }
                 }
+                if (SCALED && f == 0 && ych == 0) note_exp(d->cs, batch, ex + ey);
             }
-            T* Cb = C + batch * cb;
-#pragma unroll
-            for (int yc = 0; yc < YC; ++yc) {
-                if (yc < ny) {
-                    const int64_t y = (int64_t)ych * YC + yc;
-                    Cb[((int64_t)f << shf) | (y << shy)] = acc[yc];
-                    if (SCALED) note_max(d->cs, batch, (float)absval(acc[yc]));
-                }
+            if (++ych == nych) {
+                ych = 0;
+                ++lt;
             }
-            if (SCALED && f == 0 && ych == 0) note_exp(d->cs, batch, ex + ey);
         }
     }
 }
@@ -535,26 +590,65 @@ __global__ __launch_bounds__(kBlock) void k_permute(const T* __restrict__ in, T*
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_permute_desc(const to_perm_desc* __restrict__ descs, int ndesc) {
+    // LDS: the tile (2^nu entries), then the offsets of the 8 lowest rest bits
     extern __shared__ __align__(16) unsigned char smem[];
-    T* tile = reinterpret_cast<T*>(smem);
     const to_perm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
-    const int U = 1 << d->nu;
+    const int nu = d->nu, U = 1 << nu;
+    T* tile = reinterpret_cast<T*>(smem);
+    int64_t* lo_in = reinterpret_cast<int64_t*>(smem + ((sizeof(T) << nu) + 15) / 16 * 16);
+    int64_t* lo_out = lo_in + 256;
     const int nrest = d->nrest;
+    const int nlo = nrest < 8 ? nrest : 8;
+    for (int r = threadIdx.x; r < (1 << nlo); r += kBlock) {
+        lo_in[r] = pdep(r, d->rest_in, nlo);
+        lo_out[r] = pdep(r, d->rest_out, nlo);
+    }
+    // this thread's in-tile offsets are the same for every tile
+    constexpr int MAXE = 1024 / kBlock;  // U <= 1024
+    int64_t in_off[MAXE], out_off[MAXE];
+    int ulds[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        const int u = threadIdx.x + kBlock * i;
+        in_off[i] = u < U ? pdep(u, d->u_in, nu) : 0;
+        const int v = u;
+        int uu = 0;
+        for (int j = 0; j < nu; ++j) uu |= ((v >> j) & 1) << d->v2u[j];
+        ulds[i] = uu;
+        out_off[i] = v < U ? pdep(v, d->v_out, nu) : 0;
+    }
+    __syncthreads();
     const int64_t ntiles = (int64_t)d->nbatch << nrest;
+    const int local = blockIdx.x - d->block0;
+    const int64_t t0 = ntiles * local / d->nblocks, t1 = ntiles * (local + 1) / d->nblocks;
     const T* in = reinterpret_cast<const T*>(d->in);
     T* out = reinterpret_cast<T*>(d->out);
-    for (int64_t t = blockIdx.x - d->block0; t < ntiles; t += d->nblocks) {
+    const int64_t rmask = (1ll << nrest) - 1;
+    int64_t hi_key = -1, hi_in = 0, hi_out = 0;
+    for (int64_t t = t0; t < t1; ++t) {
         const int64_t b = t >> nrest;
-        const int64_t r = t & ((1ll << nrest) - 1);
-        const int64_t ibase = b * d->inb + pdep(r, d->rest_in, nrest);
-        const int64_t obase = b * d->outb + pdep(r, d->rest_out, nrest);
+        const int64_t r = t & rmask;
+        const int64_t key = t >> nlo;
+        if (key != hi_key) {
+            const int64_t rh = r >> nlo;
+            hi_in = b * d->inb + (pdep(rh, d->rest_in + nlo, nrest - nlo));
+            hi_out = b * d->outb + (pdep(rh, d->rest_out + nlo, nrest - nlo));
+            hi_key = key;
+        }
+        const int rl = (int)(r & ((1 << nlo) - 1));
+        const int64_t ibase = hi_in + lo_in[rl];
+        const int64_t obase = hi_out + lo_out[rl];
         __syncthreads();
-        for (int u = threadIdx.x; u < U; u += kBlock) tile[u] = in[ibase + pdep(u, d->u_in, d->nu)];
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int u = threadIdx.x + kBlock * i;
+            if (u < U) tile[u] = in[ibase + in_off[i]];
+        }
         __syncthreads();
-        for (int v = threadIdx.x; v < U; v += kBlock) {
-            int u = 0;
-            for (int j = 0; j < d->nu; ++j) u |= ((v >> j) & 1) << d->v2u[j];
-            out[obase + pdep(v, d->v_out, d->nu)] = tile[u];
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int v = threadIdx.x + kBlock * i;
+            if (v < U) out[obase + out_off[i]] = tile[ulds[i]];
         }
     }
 }
@@ -564,7 +658,8 @@ __global__ __launch_bounds__(kBlock) void k_permute_desc(const to_perm_desc* __r
 // 4 waves per block, each owning a 32x32 quarter as 2x2 MFMA 16x16 tiles; K advances 16 at
 // a time through LDS (k-major, so the 16 lanes reading one MFMA operand row hit 16
 // consecutive LDS words).  Operand layout of the 16x16x4 instructions: A[i][k] in lane
-// i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i/4), element i%4.
+// i + 16k, B[k][j] in lane j + 16k; the result D[i][j] is in lane j + 16(i/4), element
+// i%4 for f32, and in lane j + 16(i%4), element i/4 for f64 (Mfma<T>::row).
 
 template <typename T>
 struct Mfma;
@@ -575,6 +670,8 @@ struct Mfma<double> {
     static __device__ __forceinline__ V op(double a, double b, V c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    // row of result element v held by `lane`
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 
 template <>
@@ -583,9 +680,30 @@ struct Mfma<float> {
     static __device__ __forceinline__ V op(float a, float b, V c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) * 4 + v; }
 };
 
-template <typename T, bool SCALED>
+// With WIDE, float products are summed by the MFMA over 4 K-steps (64 or 128 terms) and
+// those partial sums are accumulated in double, so a long shared range (K up to 2^15 here)
+// does not grow the float rounding error with sqrt(K); it costs registers (occupancy), so
+// the host asks for it only for long shared ranges.
+template <typename T, bool WIDE>
+constexpr bool kWideAccum = WIDE && sizeof(T) == 4;
+
+template <typename T, int N>
+__device__ __forceinline__ void flush_wide(typename Mfma<T>::V (&acc)[N][N], double (&wide)[N][N][4]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                wide[i][j][v] += (double)acc[i][j][v];
+                acc[i][j][v] = T(0);
+            }
+}
+
+template <typename T, bool SCALED, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict__ descs, int ndesc) {
     constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
     __shared__ T As[BK][BM + PAD];
@@ -612,13 +730,18 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
             sa = T(x.f); ea = x.e; sb = T(y.f); eb = y.e;
         }
         V acc[2][2];
+        double wide[2][2][4];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[i][j][v] = T(0);
-        for (int64_t k0 = 0; k0 < K; k0 += BK) {
+                for (int v = 0; v < 4; ++v) {
+                    acc[i][j][v] = T(0);
+                    wide[i][j][v] = 0.0;
+                }
+        int64_t step = 0;
+        for (int64_t k0 = 0; k0 < K; k0 += BK, ++step) {
             T ra[4], rb[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -645,6 +768,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
 #pragma unroll
                     for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
             }
+            if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 2>(acc, wide);
         }
         T* C = reinterpret_cast<T*>(d->c) + batch * d->cb;
         float mx = 0.0f;
@@ -655,9 +779,10 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
                 const int64_t n = tn * BN + wn * 32 + j * 16 + (lane & 15);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int64_t m = tm * BM + wm * 32 + i * 16 + (lane >> 4) * 4 + v;
-                    C[m * d->ldcm + n * d->ldcn] = acc[i][j][v];
-                    if (SCALED) mx = fmaxf(mx, (float)absval(acc[i][j][v]));
+                    const int64_t m = tm * BM + wm * 32 + i * 16 + Mfma<T>::row(lane, v);
+                    const T out = kWideAccum<T, WIDE> ? T(wide[i][j][v]) : acc[i][j][v];
+                    C[m * d->ldcm + n * d->ldcn] = out;
+                    if (SCALED) mx = fmaxf(mx, (float)absval(out));
                 }
             }
         }
@@ -672,7 +797,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
 // 128x128 output tiles: each wave owns a 64x64 quarter as 4x4 MFMA tiles; K advances by
 // BK (16 for double, 32 for float) with 16-byte global loads, and the next K slab is
 // fetched into registers while the current one is multiplied out of LDS.
-template <typename T, bool SCALED>
+template <typename T, bool SCALED, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restrict__ descs, int ndesc) {
     constexpr int BM = 128, BN = 128, PAD = 4;
     constexpr int VE = 16 / sizeof(T);              // elements per 16-byte load
@@ -710,19 +835,24 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
             sa = T(x.f); ea = x.e; sb = T(y.f); eb = y.e;
         }
         V acc[4][4];
+        double wide[4][4][4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[i][j][v] = T(0);
+                for (int v = 0; v < 4; ++v) {
+                    acc[i][j][v] = T(0);
+                    wide[i][j][v] = 0.0;
+                }
+        int64_t step = 0;
         L ra[NV], rb[NV];
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             ra[i] = *reinterpret_cast<const L*>(A + vrow[i] * K + vk[i]);
             rb[i] = *reinterpret_cast<const L*>(B + vrow[i] * K + vk[i]);
         }
-        for (int64_t k0 = 0; k0 < K; k0 += BK) {
+        for (int64_t k0 = 0; k0 < K; k0 += BK, ++step) {
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < NV; ++i) {
@@ -753,7 +883,9 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
 #pragma unroll
                     for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
             }
+            if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 4>(acc, wide);
         }
+
         T* C = reinterpret_cast<T*>(d->c) + batch * d->cb;
         float mx = 0.0f;
 #pragma unroll
@@ -763,9 +895,10 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
                 const int64_t n = tn * BN + wn * 64 + j * 16 + (lane & 15);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int64_t m = tm * BM + wm * 64 + i * 16 + (lane >> 4) * 4 + v;
-                    C[m * d->ldcm + n * d->ldcn] = acc[i][j][v];
-                    if (SCALED) mx = fmaxf(mx, (float)absval(acc[i][j][v]));
+                    const int64_t m = tm * BM + wm * 64 + i * 16 + Mfma<T>::row(lane, v);
+                    const T out = kWideAccum<T, WIDE> ? T(wide[i][j][v]) : acc[i][j][v];
+                    C[m * d->ldcm + n * d->ldcn] = out;
+                    if (SCALED) mx = fmaxf(mx, (float)absval(out));
                 }
             }
         }
@@ -852,10 +985,16 @@ int launch_one(const to_launch& L, hipStream_t st) {
         case TO_OP_GEMM:
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
             if (L.variant == 1)
-                hipLaunchKernelGGL((k_gemm128<T, SCALED>), grid, block, 0, st,
+                hipLaunchKernelGGL((k_gemm128<T, SCALED, false>), grid, block, 0, st,
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+            else if (L.variant == 3)
+                hipLaunchKernelGGL((k_gemm128<T, SCALED, true>), grid, block, 0, st,
                                    reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
             else if (L.variant == 0)
-                hipLaunchKernelGGL((k_gemm<T, SCALED>), grid, block, 0, st,
+                hipLaunchKernelGGL((k_gemm<T, SCALED, false>), grid, block, 0, st,
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+            else if (L.variant == 2)
+                hipLaunchKernelGGL((k_gemm<T, SCALED, true>), grid, block, 0, st,
                                    reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
             else {
                 set_err("gemm: unknown tile variant");

@@ -228,8 +228,9 @@ def parse_graph(text):
     return n, edges
 
 
-def min_degree_decomposition(num_vertices, edges):
-    """Tree decomposition from greedy minimum-degree elimination (ties: smallest id).
+def min_degree_decomposition(num_vertices, edges, seed=None):
+    """Tree decomposition from greedy minimum-degree elimination (ties: smallest id, or a
+    seeded random order of the vertices when ``seed`` is given).
 
     Eliminating v creates bag {v} + N(v) and makes N(v) a clique; v's bag is attached to
     the bag of whichever neighbour is eliminated next.  Components are chained so the
@@ -242,14 +243,21 @@ def min_degree_decomposition(num_vertices, edges):
             adj[v].add(u)
     import heapq
 
-    heap = [(len(adj[v]), v) for v in range(num_vertices)]
+    if seed is None:
+        tie = list(range(num_vertices))
+    else:
+        import random
+
+        tie = list(range(num_vertices))
+        random.Random(seed).shuffle(tie)
+    heap = [(len(adj[v]), tie[v], v) for v in range(num_vertices)]
     heapq.heapify(heap)
     eliminated = [False] * num_vertices
     order_pos = [0] * num_vertices
     bags, bag_nbrs = [], []
     pos = 0
     while heap:
-        d, v = heapq.heappop(heap)
+        d, _, v = heapq.heappop(heap)
         if eliminated[v] or d != len(adj[v]):
             continue
         eliminated[v] = True
@@ -265,7 +273,7 @@ def min_degree_decomposition(num_vertices, edges):
                 adj[nb[i]].add(nb[j])
                 adj[nb[j]].add(nb[i])
         for a in nb:
-            heapq.heappush(heap, (len(adj[a]), a))
+            heapq.heappush(heap, (len(adj[a]), tie[a], a))
     td = TreeDecomposition()
     for b in bags:
         td.add_node(b)

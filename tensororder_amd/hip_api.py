@@ -295,19 +295,24 @@ class HipAPI(BaseTensorAPI):
 
     # -- compiled execution ---------------------------------------------------------
 
-    def _compile(self, tree, tables, num_groups, total_slices):
-        """Compile and upload a plan; returns an ``Executable``."""
-        nodes = schedule.analyse(tree, tables, self._regroup)
+    def choose_batch(self, nodes, tables, total_slices):
+        """Largest slice batch W (<= max_batch) whose arena fits the device budget; raises
+        OutOfMemoryError when even one slice does not fit."""
         budget = self._budget()
         fixed = sum(t.rows.size for t in tables.values()) * self._esize
         W = max(1, min(self._max_batch, total_slices))
         while True:
             peak = schedule.peak_bytes(nodes, tables, W, self._entry_type)
             if peak + fixed <= budget:
-                break
+                return W
             if W == 1:
                 raise OutOfMemoryError
             W = max(1, W // 2)
+
+    def _compile(self, tree, tables, num_groups, total_slices):
+        """Compile and upload a plan; returns an ``Executable``."""
+        nodes = schedule.analyse(tree, tables, self._regroup)
+        W = self.choose_batch(nodes, tables, total_slices)
         prog = schedule.compile_program(tree, tables, num_groups, self._entry_type, W, self._alloc,
                                         regroup=self._regroup)
         self._upload(prog.table, prog.table_host.tobytes())
@@ -484,6 +489,8 @@ class HipAPI(BaseTensorAPI):
             keep.append(gblob)
         elif v["kind"] == "stream":
             d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
+            d.lnych = v["nych"].bit_length() - 1
+            d.stage = int(v["stage"])
             blob = self._alloc(ctypes.sizeof(d))
             self._upload(blob, bytes(d))
             launches.append(_lib.Launch(op=_lib.TO_OP_JOIN_STREAM, variant=v["yc"], count=1,

@@ -345,6 +345,84 @@ class TensorNetwork:
                 )
         return stack[0]
 
+    # -- early contraction -----------------------------------------------------------
+
+    def contract_pair(self, left_index, left_edge_map, right_index, right_edge_map, free_edges, tensor_api):
+        """Contract two tensors now; the result becomes a new built tensor wired to
+        ``free_edges`` (``tensor_network.pyx:158-178``)."""
+        left = self._nodes[left_index].build(tensor_api.create_tensor)
+        right = self._nodes[right_index].build(tensor_api.create_tensor)
+        result = tensor_api.tensordot(left, right, (left_edge_map, right_edge_map))
+        if hasattr(result, "numpy"):
+            result = result.numpy()
+        self.add_node(BuiltTensor(result))
+        result_index = len(self._nodes) - 1
+        for i, e in enumerate(free_edges):
+            self._index_lists[result_index][i] = e
+            if self._edges[e][1] == left_index or self._edges[e][1] == right_index:
+                self._edges[e][1] = result_index
+            elif self._edges[e][2] == left_index or self._edges[e][2] == right_index:
+                self._edges[e][2] = result_index
+        self._nodes[left_index] = None
+        self._nodes[right_index] = None
+        return result_index
+
+    def identify_partial(self, contraction_tree, tensor_api, contract_below):
+        """Contract every join whose operands are already built and whose result has rank
+        < contract_below; return the remaining tree over the compacted network and the map
+        old edge id -> new edge id (or a negative value) (``tensor_network.pyx:180-244``)."""
+        from .tree import TreeContext
+
+        ctx = TreeContext()
+        stack = []
+        for node in contraction_tree.iterate_postorder():
+            if node.is_leaf:
+                stack.append((True, node.tensor_index))
+                continue
+            right_created, right = stack.pop()
+            left_created, left = stack.pop()
+            if left_created and right_created and len(node.free_edges) < contract_below:
+                stack.append((True, self.contract_pair(left, node.left_edge_map, right, node.right_edge_map,
+                                                       node.free_edges, tensor_api)))
+            else:
+                if left_created:
+                    left = ctx.leaf(self, left)
+                if right_created:
+                    right = ctx.leaf(self, right)
+                stack.append((False, ctx.join(left, right)))
+        top_created, top = stack.pop()
+        if top_created:
+            top = ctx.leaf(self, top)
+        new_tid, fixed = [], 0
+        for cur in range(len(self._nodes)):
+            if self._nodes[cur] is not None:
+                self._nodes[fixed] = self._nodes[cur]
+                self._index_lists[fixed] = self._index_lists[cur]
+                new_tid.append(fixed)
+                fixed += 1
+            else:
+                new_tid.append(-1)
+        self._nodes = self._nodes[:fixed]
+        self._index_lists = self._index_lists[:fixed]
+        new_eid, fixed_e = [], 0
+        for cur in range(len(self._edges)):
+            # the reference tests the first endpoint twice (tensor_network.pyx:228); edges
+            # are retargeted by contract_pair, so both endpoints agree on survival
+            if new_tid[self._edges[cur][1]] >= 0 and new_tid[self._edges[cur][1]] >= 0:
+                self._edges[fixed_e] = [fixed_e, new_tid[self._edges[cur][1]], new_tid[self._edges[cur][2]]]
+                new_eid.append(fixed_e)
+                fixed_e += 1
+            else:
+                new_eid.append(-2)
+        self._edges = self._edges[:fixed_e]
+        for t in range(fixed):
+            self._index_lists[t] = [new_eid[e] for e in self._index_lists[t]]
+        for n in ctx.nodes:
+            if n.is_leaf:
+                n.tensor_index = new_tid[n.tensor_index]
+            n.free_edges = [new_eid[e] for e in n.free_edges]
+        return ctx.get_tree(top), new_eid
+
     # -- factoring (Factor-Tree planner) -----------------------------------------
 
     def factor_out(self, tensor_index, dim1, dim2):

@@ -22,6 +22,17 @@ class SlicedExecutionPlan:
             self.edges_to_slice
         )
 
+    def contract_small(self, below_size, tensor_api):
+        """Early contraction (``sliced_execution_plan.py:29-56``): reroot so a large tensor
+        is on top, regroup small neighbours, contract every join below ``below_size`` now,
+        and renumber the slice groups' edges."""
+        self.tree = self.tree.reroot(self.network, self.maxrank)
+        self.tree.regroup(below_size)
+        self.tree, new_eid = self.network.identify_partial(self.tree, tensor_api, below_size)
+        self.edges_to_slice = set(new_eid[e] for e in self.edges_to_slice if new_eid[e] >= 0)
+        self.groups_to_slice = [set(new_eid[e] for e in g if new_eid[e] >= 0) for g in self.groups_to_slice]
+        self.next_edge_to_slice = None
+
     def slice_at(self, edge):
         equivalent = self.network.find_equivalent_edges(edge)
         self.groups_to_slice.append(equivalent)

@@ -162,13 +162,14 @@ def factor_tree_plan(network, td, seed=None):
     return tree
 
 
-def decompose(graph_text, solver=None, max_decompositions=1):
+def decompose(graph_text, solver=None, max_decompositions=1, tie_seeds=None):
     """Tree decompositions of a PACE graph: from an external anytime solver (argv list,
     e.g. a FlowCutter binary with ``-s seed -p 100``) or, with ``solver=None``, the
-    built-in greedy min-degree decomposition."""
+    built-in greedy min-degree decomposition (one per entry of ``tie_seeds``; None =
+    deterministic ties)."""
     if solver is None:
         n, edges = parse_graph(graph_text)
-        return [min_degree_decomposition(n, edges)]
+        return [min_degree_decomposition(n, edges, seed=s) for s in (tie_seeds or [None])]
     return run_solver(solver, graph_text, max_decompositions=max_decompositions)
 
 
@@ -183,6 +184,20 @@ def plan_factor(network, td, seed=None):
     tree = factor_tree_plan(net, td, seed=seed)
     tree.treewidth = None
     return tree, net
+
+
+def candidate_plans(network, planner, tie_seeds, seed=0):
+    """One unsliced plan per built-in decomposition (random tie orders ``tie_seeds``)."""
+    from .plan import SlicedExecutionPlan
+
+    text = network.line_structure_text() if planner == "line" else network.structure_text()
+    out = []
+    for td in decompose(text, None, tie_seeds=tie_seeds):
+        width = td.width()
+        tree, net = plan_line(network, td) if planner == "line" else plan_factor(network, td, seed=seed)
+        tree.treewidth = width
+        out.append(SlicedExecutionPlan(tree, net))
+    return out
 
 
 def best_plan(network, planner, solver=None, max_decompositions=1, seed=None):

@@ -199,6 +199,7 @@ def join_geometry(lrank, rrank, lmap, rmap):
     b_free = [rrank - 1 - fb[b - 1 - t] for t in range(b)]
     a_sh = [lrank - 1 - lmap[s - 1 - t] for t in range(s)]
     b_sh = [rrank - 1 - rmap[s - 1 - t] for t in range(s)]
+    assert a_free == sorted(a_free) and b_free == sorted(b_free)
     x_is_left = (a, lrank) >= (b, rrank)
     if x_is_left:
         return dict(x_is_left=True, fx=a, fy=b, s=s, shf=b, shy=0,
@@ -207,7 +208,14 @@ def join_geometry(lrank, rrank, lmap, rmap):
                 xf=b_free, xs=b_sh, yf=a_free, ys=a_sh)
 
 
-GEMM_TILES = 128  # largest GEMM output tile edge to use (64 or 128)
+import os
+
+# largest GEMM output tile edge to use (64 or 128)
+GEMM_TILES = int(os.environ.get("TENSORORDER_GEMM_TILES", "128"))
+# stage streamed-join outputs through LDS when they form one contiguous run per tile
+STAGE_OUTPUTS = os.environ.get("TENSORORDER_STAGE_OUTPUTS", "1") != "0"
+# float GEMMs with at least this many shared bits accumulate partial sums in double
+WIDE_ACCUM_MIN_SHARED = 13
 GEMM_MIN_FREE = 6
 GEMM_MIN_SHARED = 4
 MAX_GEMM_BLOCKS = 8192
@@ -227,13 +235,28 @@ def choose_variant(g, nb, esize):
         big = GEMM_TILES == 128 and fx >= 7 and fy >= 7 and s >= (4 if esize == 8 else 5)
         tb = 7 if big else 6
         tiles = nb << (fx - tb + fy - tb)
-        return dict(kind="gemm", tile=1 if big else 0, nblocks=int(min(tiles, MAX_GEMM_BLOCKS)),
+        wide = esize == 4 and s >= WIDE_ACCUM_MIN_SHARED
+        return dict(kind="gemm", tile=(1 if big else 0) | (2 if wide else 0),
+                    nblocks=int(min(tiles, MAX_GEMM_BLOCKS)),
                     perm_x=not _canonical(g["xf"], g["xs"]), perm_y=not _canonical(g["yf"], g["ys"]))
     yc = min(16, 1 << fy)
+    stage = 0
     if fx >= 8:
         while yc > 1 and 8 * K + yc * K * esize > LDS_LIMIT:
             yc //= 2
         lds = 8 * K + yc * K * esize
+        # outputs of a tile form one contiguous run when the Y-free bits sit below the f
+        # bits and one chunk holds all of them: stage that run through LDS
+        if STAGE_OUTPUTS and g["shy"] == 0 and g["shf"] == fy and yc == (1 << fy) and fy >= 1 \
+                and lds + (256 << fy) * esize <= LDS_LIMIT:
+            stage |= 1
+            lds += (256 << fy) * esize
+        # the X tile (256 f values x all k) is one contiguous run when every shared bit of
+        # X lies below bit 8 + s; stage it through LDS when lanes would otherwise stride
+        if STAGE_OUTPUTS and s >= 1 and max(g["xs"]) < 8 + s and lds + (256 << s) * esize <= LDS_LIMIT \
+                and (256 << s) * esize <= 32 * 1024:
+            stage |= 2
+            lds += (256 << s) * esize
     else:
         lds = 8 * K
     nych = (1 << fy) // yc
@@ -244,7 +267,7 @@ def choose_variant(g, nb, esize):
             ntiles = nb * (1 << (fx - 8)) * nych
         else:
             ntiles = -(-(nb << fx) // 256) * nych
-        return dict(kind="stream", yc=yc, nych=nych, lds=lds,
+        return dict(kind="stream", yc=yc, nych=nych, lds=lds, stage=stage,
                     nblocks=int(min(ntiles, MAX_STREAM_BLOCKS)))
     nout = nb << (fx + fy)
     nsplit = 1
@@ -321,7 +344,8 @@ def perm_desc(src, dst, rank, out_pos, nbatch, stride, esize):
     ntiles = nbatch << len(rest_in)
     d.block0 = 0
     d.nblocks = int(min(ntiles, 8192))
-    return d, d.nblocks, esize << len(u_in)
+    lds = ((esize << len(u_in)) + 15) // 16 * 16 + 2 * 256 * 8
+    return d, d.nblocks, lds
 
 
 def _pos_array(vals):
@@ -467,6 +491,8 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
     d.yf_pos, d.ys_pos = _pos_array(g["yf"]), _pos_array(g["ys"])
     if v["kind"] == "stream":
         d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
+        d.lnych = v["nych"].bit_length() - 1
+        d.stage = int(v["stage"])
         out.append(("stream", _lib.TO_OP_JOIN_STREAM, v["yc"], v["lds"], d, v["nblocks"]))
         return out
     d.partial = abase + scratch["partial"][0]

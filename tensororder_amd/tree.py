@@ -121,6 +121,39 @@ class TreeContext:
                 combined = self.join(combined, self.leaf(network, tensor_id))
         return self.join(combined, tree)
 
+    def group_if_below(self, upper, upper_left, lower_left, if_below):
+        """At (A (B C)) with A = upper's ``upper_left`` child and B = the lower join's
+        ``lower_left`` child, regroup to (C (A B)) when rank(A B) < if_below
+        (``contraction_tree.pyx:324-380``).  Returns 1 if regrouped."""
+        un = self.nodes[upper]
+        if upper_left:
+            chosen, lower = un.left, un.right
+        else:
+            chosen, lower = un.right, un.left
+        ln = self.nodes[lower]
+        if lower_left:
+            lchosen, lother = ln.left, ln.right
+        else:
+            lchosen, lother = ln.right, ln.left
+        a, b = self.nodes[chosen].free_edges, self.nodes[lchosen].free_edges
+        rank = len(a) + len(b)
+        for e in a:
+            if e in b:
+                rank -= 2
+        if rank < if_below:
+            if upper_left:
+                un.left = lother
+            else:
+                un.right = lother
+            if lower_left:
+                ln.right = chosen
+            else:
+                ln.left = chosen
+            self.compute_join_properties(lower)
+            self.compute_join_properties(upper)
+            return 1
+        return 0
+
     def postorder_ids(self, root):
         out = []
         stack = [(root, False)]
@@ -284,6 +317,61 @@ class ContractionTree:
                 processed.append((node, True))
                 processed.append((node.right, False))
                 processed.append((node.left, False))
+
+    def reroot(self, network, large_rank):
+        """A new tree whose top joins the first subtree of max rank >= large_rank with the
+        remaining pieces along its path (``contraction_tree.pyx:49-89``)."""
+        ctx = TreeContext()
+        stack, path_up = [], []
+        for node in self.iterate_postorder():
+            if node.is_leaf:
+                stack.append(ctx.leaf(network, node.tensor_index))
+                continue
+            right = stack.pop()
+            left = stack.pop()
+            if not path_up and node.maxrank >= large_rank:
+                path_up.append(ctx.join(left, right))
+                stack.append(-1)
+            elif right == -1:
+                path_up.append(left)
+                stack.append(-1)
+            elif left == -1:
+                path_up.append(right)
+                stack.append(-1)
+            else:
+                stack.append(ctx.join(left, right))
+        if not path_up:
+            return ctx.get_tree(stack[0])
+        top = path_up[-1]
+        for i in range(len(path_up) - 1, 0, -1):
+            top = ctx.join(path_up[i - 1], top)
+        return ctx.get_tree(top)
+
+    def regroup(self, rank_limit):
+        """Regroup (A (B C)) with A, B and (A B) small and C large into (C (A B)), in place
+        (``contraction_tree.pyx:91-135``)."""
+        ctx = self._context
+        stack = []
+        for nid in ctx.postorder_ids(self._node):
+            n = ctx.nodes[nid]
+            if n.is_leaf:
+                stack.append((None, n.maxrank < rank_limit))
+                continue
+            r_sol, r_small = stack.pop()
+            l_sol, l_small = stack.pop()
+            if r_sol is not None and l_small:
+                if ctx.group_if_below(nid, True, r_sol, rank_limit) == 1:
+                    l_small, r_small = False, True
+            elif l_sol is not None and r_small:
+                if ctx.group_if_below(nid, False, l_sol, rank_limit) == 1:
+                    l_small, r_small = True, False
+            if l_small and not r_small:
+                sol = True
+            elif r_small and not l_small:
+                sol = False
+            else:
+                sol = None
+            stack.append((sol, n.maxrank < rank_limit))
 
     def sort_small(self):
         """Swap operands so the lower-rank one is on the right; recompute join

@@ -259,3 +259,17 @@ def test_run_reports_errors_as_none():
 def test_base_api_rejects_unknown_arguments():
     with pytest.raises(ValueError):
         api.BaseTensorAPI().add_argument("nope", 1)
+
+
+@pytest.mark.parametrize("early", [4, 6, 9])
+def test_early_contraction_preserves_the_count(early):
+    f = parse_dimacs_text(random_3cnf(20, 48, 5), "cachet")
+    plan = planners.best_plan(network.cnf_count(f), "factor", solver=None, seed=0)
+    ALL_SLICERS["greedy_mem"].slice_until(plan, slices=3)
+    want = numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan))
+    n_before = len(plan.network)
+    plan.contract_small(early, NumpyOps())
+    assert len(plan.network) < n_before
+    assert is_tree_complete(plan.tree, plan.network)
+    got = numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan))
+    assert got == pytest.approx(want, rel=1e-12)
