@@ -40,33 +40,73 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def kernel_symbol(L, scaled):
+    """Name of the HIP kernel one launch record runs, as a profiler lists it."""
+    from tensororder_amd import _lib
+
+    T = "float" if scaled else "double"
+    S = "true" if scaled else "false"
+    op, v = L.op, L.variant
+    if op == _lib.TO_OP_GEMM:
+        return "%s<%s, %s, %s>" % ("k_gemm128" if v & 1 else "k_gemm", T, S, "true" if v & 2 else "false")
+    if op == _lib.TO_OP_JOIN_STREAM:
+        return "k_join_stream<%s, %s, %d>" % (T, S, v)
+    if op == _lib.TO_OP_PERMUTE:
+        return "%s<%s>" % ("k_permute_vec_desc" if v == 1 else "k_permute_desc", T)
+    names = {_lib.TO_OP_LEAF_GATHER: "k_leaf_gather", _lib.TO_OP_JOIN_SPLITK: "k_join_splitk",
+             _lib.TO_OP_JOIN_FINISH: "k_join_finish", _lib.TO_OP_ROOT_REDUCE: "k_root_reduce",
+             _lib.TO_OP_ROOT_STORE: "k_root_store"}
+    if op in names:
+        return "%s<%s, %s>" % (names[op], T, S)
+    return {_lib.TO_OP_SCALE_CLEAR: "k_scale_clear", _lib.TO_OP_WAVE_ADVANCE: "k_wave_advance",
+            _lib.TO_OP_STEP_BEGIN: "k_step_begin", _lib.TO_OP_BATCH_BEGIN: "k_batch_begin"}.get(op, "op%d" % op)
+
+
+def roofline_of(name, k):
+    """Roofline object of one kernel symbol from its summed launches of one step."""
+    if k["flops"] and k["flops"] / max(k["bytes"], 1) > 20 and name.startswith("k_gemm"):
+        peak = FP32_PEAK_TFLOPS if "<float" in name else FP64_PEAK_TFLOPS
+        achieved = k["flops"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
+        return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None, "kernel": name, "launches_per_step": k["launches"],
+                "kernel_ms_per_step": k["ms"], "avg_launch_us": 1e3 * k["ms"] / max(k["launches"], 1),
+                "algorithmic_flops_per_step": k["flops"]}
+    achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name, "launches_per_step": k["launches"],
+            "kernel_ms_per_step": k["ms"], "avg_launch_us": 1e3 * k["ms"] / max(k["launches"], 1),
+            "algorithmic_bytes_per_step": k["bytes"]}
+
+
 def per_kernel_times(api, exe):
-    """Device ms of every launch of one step, by kernel class, with algorithmic bytes."""
+    """Device ms of every launch of one step, with algorithmic bytes and flops, summed by
+    kernel class and by kernel symbol (a symbol is one template instance)."""
     import ctypes
 
-    from tensororder_amd import _lib, schedule
+    from tensororder_amd import _lib
 
     prog = exe.prog
-    esize = api.get_entry_size()
+    scaled = api._entry_type == _lib.TO_F32S
     launches = list(exe.step_launches)
     info = [("begin", None)] + list(prog.prologue_info)
+    work = [(0, 0)] + list(prog.prologue_work)
     for _ in range(exe.nwaves):
         launches += exe.wave_launches
         info += prog.wave_info
+        work += prog.wave_work
     arr = _lib.launch_array(launches)
     ms = (ctypes.c_float * len(launches))()
     _lib.check(api._lib.to_run_timed(ctypes.addressof(arr), len(launches), api._entry_type,
                                      ctypes.c_void_p(api._stream.cuda_stream), ms), "to_run_timed")
-    classes = {}
-    for (kind, nidx), t in zip(info, ms):
-        c = classes.setdefault(kind, {"ms": 0.0, "launches": 0, "bytes": 0, "flops": 0})
-        c["ms"] += t
-        c["launches"] += 1
-        if nidx is not None and kind in ("stream", "splitk", "gemm"):
-            b, f = schedule.join_traffic(prog.nodes[nidx], prog.nodes, prog.nbatch, esize)
+    classes, symbols = {}, {}
+    for L, (kind, _nidx), (b, f), t in zip(launches, info, work, ms):
+        for key, table in ((kind, classes), (kernel_symbol(L, scaled), symbols)):
+            c = table.setdefault(key, {"ms": 0.0, "launches": 0, "bytes": 0, "flops": 0})
+            c["ms"] += t
+            c["launches"] += 1
             c["bytes"] += b
             c["flops"] += f
-    return classes
+    return classes, symbols
 
 
 def cpu_baseline(doc, budget_s, full=False):
@@ -121,6 +161,73 @@ def cpu_baseline(doc, budget_s, full=False):
                       "%.1f s per instance; BLAS threads %d" % (total, done, dt, total, per_instance, threads)}
 
 
+def copy_calibration():
+    """A plain device copy of known size: 512 MiB read, 512 MiB written per launch.  The
+    profiler's FETCH_SIZE / WRITE_SIZE of this launch give the counters' scale."""
+    import torch
+
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+
+
+def permute_sweep(rank=26, reps=10):
+    """Standalone bit-permutation throughput (to_permute) on rank-``rank`` tensors: for each
+    entry size and permutation, algorithmic bytes (read once + write once) per second."""
+    import ctypes
+    import random
+
+    import torch
+
+    from tensororder_amd import _lib
+
+    lib = _lib.load()
+    rng = random.Random(0)
+    perms = {
+        "reverse": list(range(rank))[::-1],
+        "rotate_half": [(i + rank // 2) % rank for i in range(rank)],
+        "low8_to_top": list(range(8, rank)) + list(range(8)),
+        "random": rng.sample(range(rank), rank),
+    }
+    rows = []
+    stream = torch.cuda.current_stream()
+    for esize, dt in ((4, torch.float32), (8, torch.float64)):
+        a = torch.rand(1 << rank, dtype=dt, device="cuda")
+        b = torch.empty_like(a)
+        for name, perm in perms.items():
+            p = (ctypes.c_int32 * rank)(*perm)
+
+            def go():
+                _lib.check(lib.to_permute(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), rank,
+                                          ctypes.addressof(p), esize, 1, ctypes.c_void_p(stream.cuda_stream)),
+                           "to_permute")
+
+            go()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                go()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / reps
+            gbs = 2 * esize * (1 << rank) / (us * 1e-6) / 1e9
+            # numpy.transpose semantics: output axis k is input axis perm[k]
+            j = torch.arange(1 << rank, device="cuda")
+            src = torch.zeros_like(j)
+            for k in range(rank):
+                src |= ((j >> (rank - 1 - k)) & 1) << (rank - 1 - perm[k])
+            ok = bool(torch.equal(b, a[src]))
+            del j, src
+            rows.append({"entry_bytes": esize, "perm": name, "us": us, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS,
+                         "correct": ok})
+            log(rows[-1])
+    print(json.dumps({"permute_sweep": rows, "rank": rank}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,7 +237,17 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-full", action="store_true", help="time the full unsliced oracle contraction")
+    ap.add_argument("--permute-sweep", action="store_true",
+                    help="only time standalone bit permutations of rank-26 tensors")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 512 MiB three times (reference for HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.permute_sweep:
+        permute_sweep()
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the executor has no multi-GPU path in this build "
               "(slices would shard across GPUs with one FP64 partial sum each); use --gpus 1",
@@ -161,8 +278,8 @@ def main():
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         log(name, "steps", args.steps, "s", round(dt, 4), "count", res.value, "log10", res.log10)
-        classes = per_kernel_times(api, exe)
-        results[name] = dict(api=api, exe=exe, dt=dt, res=res, classes=classes)
+        classes, symbols = per_kernel_times(api, exe)
+        results[name] = dict(api=api, exe=exe, dt=dt, res=res, classes=classes, symbols=symbols)
         log(name, "kernels", {k: (round(v["ms"], 3), v["launches"]) for k, v in classes.items()})
 
     r32, r64 = results["fp32s"], results["fp64"]
@@ -170,21 +287,13 @@ def main():
     rel_64_ref = abs(r64["res"].value - expected) / abs(expected) if expected else None
     rel_32_ref = abs(r32["res"].value - expected) / abs(expected) if expected else None
 
-    classes = r32["classes"]
-    dom_kind = max(classes, key=lambda k: classes[k]["ms"])
-    dom = classes[dom_kind]
-    if dom_kind == "gemm":
-        achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12 if dom["ms"] > 0 else 0.0
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_gemm",
-                    "launches_per_step": dom["launches"], "kernel_ms_per_step": dom["ms"],
-                    "algorithmic_flops_per_step": dom["flops"]}
-    else:
-        achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 and dom["bytes"] else 0.0
-        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_join_" + dom_kind,
-                    "launches_per_step": dom["launches"], "kernel_ms_per_step": dom["ms"],
-                    "algorithmic_bytes_per_step": dom["bytes"]}
+    classes, symbols = r32["classes"], r32["symbols"]
+    dom_name = max(symbols, key=lambda k: symbols[k]["ms"])
+    dom = symbols[dom_name]
+    roofline = roofline_of(dom_name, dom)
+    # the permute kernel's own HBM roofline (it is HBM-bound whatever dominates)
+    perm = {k: v for k, v in symbols.items() if k.startswith("k_permute")}
+    perm_roofline = roofline_of(next(iter(perm)), perm[next(iter(perm))]) if perm else None
 
     out = {
         "metric": METRIC,
@@ -209,8 +318,9 @@ def main():
         "rel_err_fp32s_vs_oracle": rel_32_ref,
         "log10_count": r64["res"].log10,
         "roofline": roofline,
-        "kernel_classes": {k: {"ms": v["ms"], "launches": v["launches"], "bytes": v["bytes"],
-                               "flops": v["flops"]} for k, v in classes.items()},
+        "permute_roofline": perm_roofline,
+        "kernel_classes": classes,
+        "kernel_symbols": symbols,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(doc, args.cpu_budget, args.cpu_full)

@@ -55,7 +55,9 @@ extern "C" {
                                 aux2: to_accum*, aux3: wave params (int64[2]) */
 #define TO_OP_WAVE_ADVANCE 7 /* aux3: wave params; adds iaux0 to params[0] */
 #define TO_OP_STEP_BEGIN 8   /* aux2: to_accum* (zeroed), aux3: wave params (params[0] = 0) */
-#define TO_OP_PERMUTE 9      /* descs: to_perm_desc[count] */
+#define TO_OP_PERMUTE 9      /* descs: to_perm_desc[count]; variant 1: 16-byte chunk kernel,
+                                which needs nu == 13 (float) / 12 (double) and 16-byte
+                                aligned buffers and strides; variant 0: nu <= 12 */
 #define TO_OP_GEMM 10        /* descs: to_gemm_desc[count]; variant bit 0: 128x128 output
                                 tiles (mlog, nlog >= 7) instead of 64x64 (>= 6); bit 1: float
                                 partial sums accumulated in double every 4 K-steps */

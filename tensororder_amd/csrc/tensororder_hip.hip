@@ -16,6 +16,7 @@
 //  * FP32 mode keeps one power-of-two exponent per tensor and batch item; consumers
 //    rescale on load, producers fold max |entry| into an atomicMax on its float bits.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include <stdint.h>
 #include <stdio.h>
@@ -43,12 +44,31 @@ int hip_fail(hipError_t e, const char* where) {
 // ---------------------------------------------------------------------------------
 // device helpers
 
+// Device addresses arrive as integers in the descriptors; casting them to global-memory
+// pointers (address space 1) makes the compiler emit global_* instead of flat_* memory
+// instructions, so LDS waits do not also wait for outstanding HBM loads.
+#define TO_GLOBAL __attribute__((address_space(1)))
+
+template <typename T>
+__device__ __forceinline__ TO_GLOBAL T* gptr(uint64_t p) {
+    return (TO_GLOBAL T*)p;
+}
+
 __device__ __forceinline__ int64_t pdep(uint64_t v, const int8_t* pos, int n) {
     int64_t r = 0;
     for (int i = 0; i < n; ++i) r |= (int64_t)((v >> i) & 1ull) << pos[i];
     return r;
 }
 
+// a value known to be the same in every lane, moved to scalar registers
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// index of the descriptor whose block range holds block bid; the result is the same for
+// every lane of the block, and is marked wave-uniform so descriptor fields load as scalars
 template <class D>
 __device__ __forceinline__ int find_desc(const D* __restrict__ descs, int n, int bid) {
     int lo = 0, hi = n - 1;
@@ -57,7 +77,7 @@ __device__ __forceinline__ int find_desc(const D* __restrict__ descs, int n, int
         if (descs[mid].block0 <= bid) lo = mid;
         else hi = mid - 1;
     }
-    return lo;
+    return __builtin_amdgcn_readfirstlane(lo);
 }
 
 // frexp exponent of a non-negative float given by its bits (0 for zero).
@@ -99,18 +119,18 @@ struct OpScale {
 };
 
 __device__ __forceinline__ OpScale op_scale(uint64_t recs, int64_t stride, int64_t b) {
-    const to_scale_rec* r = reinterpret_cast<const to_scale_rec*>(recs) + b * stride;
+    const TO_GLOBAL to_scale_rec* r = gptr<const to_scale_rec>(recs) + b * stride;
     const int e = scale_exp(r->maxbits);
     return OpScale{ldexpf(1.0f, -e), r->exp + e};
 }
 
 __device__ __forceinline__ void note_max(uint64_t recs, int64_t b, float m) {
-    to_scale_rec* r = reinterpret_cast<to_scale_rec*>(recs) + b;
-    atomicMax(&r->maxbits, __float_as_uint(m));
+    TO_GLOBAL to_scale_rec* r = gptr<to_scale_rec>(recs) + b;
+    atomicMax((uint32_t*)&r->maxbits, __float_as_uint(m));
 }
 
 __device__ __forceinline__ void note_exp(uint64_t recs, int64_t b, int64_t e) {
-    to_scale_rec* r = reinterpret_cast<to_scale_rec*>(recs) + b;
+    TO_GLOBAL to_scale_rec* r = gptr<to_scale_rec>(recs) + b;
     r->exp = e;
 }
 
@@ -131,9 +151,9 @@ __global__ __launch_bounds__(kBlock) void k_leaf_gather(const to_leaf_desc* __re
     const int64_t total = (int64_t)d->nbatch << rank;
     const int64_t begin = wave ? wave[0] : 0;
     const int G = d->ngroups;
-    const int64_t* coef = reinterpret_cast<const int64_t*>(d->coef);
-    const T* src = reinterpret_cast<const T*>(d->src);
-    T* dst = reinterpret_cast<T*>(d->dst);
+    const TO_GLOBAL int64_t* coef = gptr<const int64_t>(d->coef);
+    const TO_GLOBAL T* src = gptr<const T>(d->src);
+    TO_GLOBAL T* dst = gptr<T>(d->dst);
     const int64_t stride = (int64_t)d->nblocks * kBlock;
     for (int64_t i = (int64_t)(blockIdx.x - d->block0) * kBlock + threadIdx.x; i < total; i += stride) {
         const int64_t b = i >> rank;
@@ -143,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf_gather(const to_leaf_desc* __re
         for (int g = 0; g < G; ++g) row += coef[g] * ((sidx >> (G - 1 - g)) & 1);
         dst[i] = src[row * E + e];
         if (SCALED && e == 0 && d->rec) {
-            to_scale_rec* r = reinterpret_cast<to_scale_rec*>(d->rec) + b;
+            TO_GLOBAL to_scale_rec* r = gptr<to_scale_rec>(d->rec) + b;
             r->exp = d->exp0;
             r->maxbits = __float_as_uint(1.0f);
         }
@@ -165,9 +185,9 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
     const int lnych = d->lnych;
     const int nych = 1 << lnych;
     const int nbatch = d->nbatch;
-    const T* __restrict__ X = reinterpret_cast<const T*>(d->x);
-    const T* __restrict__ Y = reinterpret_cast<const T*>(d->y);
-    T* __restrict__ C = reinterpret_cast<T*>(d->c);
+    const TO_GLOBAL T* __restrict__ X = gptr<const T>(d->x);
+    const TO_GLOBAL T* __restrict__ Y = gptr<const T>(d->y);
+    TO_GLOBAL T* __restrict__ C = gptr<T>(d->c);
     const int64_t xb = d->xb, yb = d->yb, cb = d->cb;
     const int shf = d->shf, shy = d->shy;
     const int NY = 1 << fy;
@@ -218,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
             }
             if (key != staged) {
                 __syncthreads();
-                const T* Yb = Y + batch * yb;
+                const auto Yb = Y + batch * yb;
                 for (int i = tid; i < YC * K; i += kBlock) {
                     const int yc = i >> s, k = i & (K - 1);
                     T v = T(0);
@@ -229,11 +249,11 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                 staged = key;
             }
             const int64_t f = (ft << 8) | tid;
-            const T* Xp = X + batch * xb + xhi + xlo;
+            const auto Xp = X + batch * xb + xhi + xlo;
             if (stage_in) {
                 // the tile's X entries are the contiguous run of 2^(8+s) entries at the
                 // high-bit offset: load it linearly, then read this thread's entries in LDS
-                const T* src = X + batch * xb + xhi;
+                const auto src = X + batch * xb + xhi;
                 __syncthreads();
                 for (int i = tid; i < (kBlock << s); i += kBlock) xtile[i] = src[i];
                 __syncthreads();
@@ -258,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                     }
                 }
             }
-            T* Cb = C + batch * cb;
+            auto Cb = C + batch * cb;
             float m = 0.0f;
             if (SCALED) {
 #pragma unroll
@@ -273,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                 for (int yc = 0; yc < YC; ++yc)
                     if (yc < NY) otile[tid * NY + yc] = acc[yc];
                 __syncthreads();
-                T* dst = Cb + (ft << (8 + fy));
+                auto dst = Cb + (ft << (8 + fy));
                 for (int i = tid; i < (kBlock << fy); i += kBlock) dst[i] = otile[i];
             } else {
 #pragma unroll
@@ -323,8 +343,8 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                     const OpScale c = op_scale(d->ys, d->ysb, batch);
                     sx = a.f; ex = a.e; sy = c.f; ey = c.e;
                 }
-                const T* Xp = X + batch * xb + xoff;
-                const T* Yb = Y + batch * yb;
+                const auto Xp = X + batch * xb + xoff;
+                const auto Yb = Y + batch * yb;
                 int64_t yoff[YC];
 #pragma unroll
                 for (int yc = 0; yc < YC; ++yc) yoff[yc] = (yc < ny) ? pdep(ych * YC + yc, d->yf_pos, fy) : 0;
@@ -338,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                         if (yc < ny) acc[yc] += xv * (Yb[yoff[yc] + ky[k]] * T(sy));
                     }
                 }
-                T* Cb = C + batch * cb;
+                auto Cb = C + batch * cb;
 #pragma unroll
                 for (int yc = 0; yc < YC; ++yc) {
                     if (yc < ny) {
@@ -373,8 +393,8 @@ __global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __re
     const int64_t batch = o >> (fx + fy);
     const int64_t f = (o >> fy) & ((1ll << fx) - 1);
     const int64_t y = o & ((1ll << fy) - 1);
-    const T* X = reinterpret_cast<const T*>(d->x) + batch * d->xb + pdep(f, d->xf_pos, fx);
-    const T* Y = reinterpret_cast<const T*>(d->y) + batch * d->yb + pdep(y, d->yf_pos, fy);
+    const TO_GLOBAL T* X = gptr<const T>(d->x) + batch * d->xb + pdep(f, d->xf_pos, fx);
+    const TO_GLOBAL T* Y = gptr<const T>(d->y) + batch * d->yb + pdep(y, d->yf_pos, fy);
     float sx = 1.0f, sy = 1.0f;
     if (SCALED) {
         sx = op_scale(d->xs, d->xsb, batch).f;
@@ -393,7 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __re
         sum = (X[pdep(tid, d->xs_pos, s)] * T(sx)) * (Y[pdep(tid, d->ys_pos, s)] * T(sy));
     }
     sum = block_sum(sum, red);
-    if (tid == 0) reinterpret_cast<T*>(d->partial)[o * nsplit + sp] = sum;
+    if (tid == 0) gptr<T>(d->partial)[o * nsplit + sp] = sum;
 }
 
 template <typename T, bool SCALED>
@@ -402,8 +422,8 @@ __global__ __launch_bounds__(kBlock) void k_join_finish(const to_join_desc* __re
     const int fx = d->fx, fy = d->fy;
     const int nsplit = d->nsplit;
     const int64_t nout = (int64_t)d->nbatch << (fx + fy);
-    const T* partial = reinterpret_cast<const T*>(d->partial);
-    T* C = reinterpret_cast<T*>(d->c);
+    const TO_GLOBAL T* partial = gptr<const T>(d->partial);
+    TO_GLOBAL T* C = gptr<T>(d->c);
     const int64_t stride = (int64_t)d->nblocks * kBlock;
     for (int64_t o = (int64_t)(blockIdx.x - d->block0) * kBlock + threadIdx.x; o < nout; o += stride) {
         T sum = T(0);
@@ -443,12 +463,12 @@ __global__ __launch_bounds__(kBlock) void k_root_store(const to_root_desc* __res
     const int64_t now = (int64_t)wall_clock64();
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < ndesc; i += gridDim.x * kBlock) {
         const to_root_desc& d = descs[i];
-        const double v = (double)*reinterpret_cast<const T*>(d.root);
+        const double v = (double)*gptr<const T>(d.root);
         int e = 0;
         const double m = frexp(v, &e);
-        to_accum* acc = reinterpret_cast<to_accum*>(d.acc);
+        TO_GLOBAL to_accum* acc = gptr<to_accum>(d.acc);
         acc->mant = m;
-        acc->exp = m == 0.0 ? 0 : (int64_t)e + (SCALED ? reinterpret_cast<const to_scale_rec*>(d.rec)->exp : 0);
+        acc->exp = m == 0.0 ? 0 : (int64_t)e + (SCALED ? gptr<const to_scale_rec>(d.rec)->exp : 0);
         acc->count = 1;
         acc->pad = now;
     }
@@ -547,52 +567,52 @@ __global__ __launch_bounds__(kBlock) void k_root_reduce(const T* __restrict__ ro
 
 // ---------------------------------------------------------------------------------
 // bit-permutation of all-dimension-2 tensors, LDS-tiled: a tile spans the union U of
-// the m lowest input bits and the input bits that land on the m lowest output bits,
-// so the tile is read with the input's low bits contiguous and written with the
-// output's low bits contiguous.
+// the m lowest input bits and the input bits that land on the m lowest output bits
+// (up to 2^12 entries), so the tile is read with the input's low bits contiguous and
+// written with the output's low bits contiguous.  Each thread moves up to kPermEpt
+// entries of a tile; the next tile's loads are issued before the current tile is stored.
 
-struct PermPlan {
-    int rank, nu, nrest, pad;
-    int8_t u_in[16];     // input bit position of tile bit j (u order: input-ascending)
-    int8_t v_out[16];    // output bit position of tile bit j (v order: output-ascending)
-    int8_t v2u[16];      // tile bit j of v order is tile bit v2u[j] of u order
-    int8_t rest_in[32];  // input position of rest bit j
-    int8_t rest_out[32]; // output position of rest bit j
+constexpr int kPermEpt = 16;  // 2^12 tile entries / 256 threads
+
+// LDS address of tile entry u: the tile bits that the 6 lowest output bits come from
+// (the lanes of a storing wave vary exactly those) are folded by xor into the low 6
+// address bits they do not already cover, so loading lanes (consecutive u) and storing
+// lanes both hit distinct banks.  The map is linear over xor and a bijection of the tile.
+struct PermSwizzle {
+    int n;
+    int src[6], dst[6];
 };
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_permute(const T* __restrict__ in, T* __restrict__ out,
-                                                    PermPlan p, int64_t batch) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    T* tile = reinterpret_cast<T*>(smem);
-    const int U = 1 << p.nu;
-    const int64_t n = 1ll << p.rank;
-    const int64_t ntiles = batch << p.nrest;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int64_t b = t >> p.nrest;
-        const int64_t r = t & ((1ll << p.nrest) - 1);
-        const int64_t ibase = b * n + pdep(r, p.rest_in, p.nrest);
-        const int64_t obase = b * n + pdep(r, p.rest_out, p.nrest);
-        __syncthreads();
-        for (int u = threadIdx.x; u < U; u += kBlock) tile[u] = in[ibase + pdep(u, p.u_in, p.nu)];
-        __syncthreads();
-        for (int v = threadIdx.x; v < U; v += kBlock) {
-            int u = 0;
-            for (int j = 0; j < p.nu; ++j) u |= ((v >> j) & 1) << p.v2u[j];
-            out[obase + pdep(v, p.v_out, p.nu)] = tile[u];
+__device__ inline PermSwizzle perm_swizzle(const to_perm_desc* d) {
+    const int nb = d->nu < 6 ? d->nu : 6;
+    int used = 0;
+    for (int j = 0; j < nb; ++j)
+        if (d->v2u[j] < nb) used |= 1 << d->v2u[j];
+    PermSwizzle z;
+    z.n = 0;
+    int pos = 0;
+    for (int j = 0; j < nb; ++j) {
+        const int b = d->v2u[j];
+        if (b >= nb) {
+            while ((used >> pos) & 1) ++pos;
+            z.src[z.n] = b;
+            z.dst[z.n] = pos;
+            used |= 1 << pos;
+            ++z.n;
         }
     }
+    return z;
 }
 
-// ---------------------------------------------------------------------------------
-// descriptor-driven bit permutation (same tiling as k_permute), used to bring the two
-// operands of a dense join into row-major [free][shared] form for the MFMA product
+__device__ inline int perm_swz(int u, const PermSwizzle& z) {
+    int a = u;
+    for (int i = 0; i < z.n; ++i) a ^= ((u >> z.src[i]) & 1) << z.dst[i];
+    return a;
+}
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_permute_desc(const to_perm_desc* __restrict__ descs, int ndesc) {
-    // LDS: the tile (2^nu entries), then the offsets of the 8 lowest rest bits
-    extern __shared__ __align__(16) unsigned char smem[];
-    const to_perm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+__device__ __forceinline__ void permute_tiles(const to_perm_desc* __restrict__ d, int local, int nblocks,
+                                              unsigned char* smem) {
     const int nu = d->nu, U = 1 << nu;
     T* tile = reinterpret_cast<T*>(smem);
     int64_t* lo_in = reinterpret_cast<int64_t*>(smem + ((sizeof(T) << nu) + 15) / 16 * 16);
@@ -603,54 +623,272 @@ __global__ __launch_bounds__(kBlock) void k_permute_desc(const to_perm_desc* __r
         lo_in[r] = pdep(r, d->rest_in, nlo);
         lo_out[r] = pdep(r, d->rest_out, nlo);
     }
-    // this thread's in-tile offsets are the same for every tile
-    constexpr int MAXE = 1024 / kBlock;  // U <= 1024
-    int64_t in_off[MAXE], out_off[MAXE];
-    int ulds[MAXE];
+    const PermSwizzle z = perm_swizzle(d);
+    // entry e = tid + 256 i of a tile: its input offset, output offset and LDS addresses
+    // are linear in the bits of e (sums / xors over disjoint bits), so each splits into a
+    // per-thread part (tid) and a part of i shared by the block
+    auto umap = [&](int e) {
+        int u = 0;
+        for (int j = 0; j < nu; ++j) u |= ((e >> j) & 1) << d->v2u[j];
+        return u;
+    };
+    const int tid = threadIdx.x;
+    const int in_lo = (int)pdep(tid, d->u_in, nu), out_lo = (int)pdep(tid, d->v_out, nu);
+    const int wa_lo = perm_swz(tid, z), ra_lo = perm_swz(umap(tid), z);
+    // images of the 4 bits of i (entry bits 8..11); the part of entry i is their sum/xor
+    int bin[4], bout[4], bwa[4], bra[4];
 #pragma unroll
-    for (int i = 0; i < MAXE; ++i) {
-        const int u = threadIdx.x + kBlock * i;
-        in_off[i] = u < U ? pdep(u, d->u_in, nu) : 0;
-        const int v = u;
-        int uu = 0;
-        for (int j = 0; j < nu; ++j) uu |= ((v >> j) & 1) << d->v2u[j];
-        ulds[i] = uu;
-        out_off[i] = v < U ? pdep(v, d->v_out, nu) : 0;
+    for (int k = 0; k < 4; ++k) {
+        const int e = kBlock << k;
+        bin[k] = (int)pdep(e, d->u_in, nu);
+        bout[k] = (int)pdep(e, d->v_out, nu);
+        bwa[k] = perm_swz(e, z);
+        bra[k] = perm_swz(umap(e), z);
     }
+    auto hi_sum = [](const int* b, int i) {
+        return ((i & 1) ? b[0] : 0) + ((i & 2) ? b[1] : 0) + ((i & 4) ? b[2] : 0) + ((i & 8) ? b[3] : 0);
+    };
+    auto hi_xor = [](const int* b, int i) {
+        return ((i & 1) ? b[0] : 0) ^ ((i & 2) ? b[1] : 0) ^ ((i & 4) ? b[2] : 0) ^ ((i & 8) ? b[3] : 0);
+    };
     __syncthreads();
     const int64_t ntiles = (int64_t)d->nbatch << nrest;
-    const int local = blockIdx.x - d->block0;
-    const int64_t t0 = ntiles * local / d->nblocks, t1 = ntiles * (local + 1) / d->nblocks;
-    const T* in = reinterpret_cast<const T*>(d->in);
-    T* out = reinterpret_cast<T*>(d->out);
+    const int64_t t0 = ntiles * local / nblocks, t1 = ntiles * (local + 1) / nblocks;
+    const TO_GLOBAL T* in = gptr<const T>(d->in);
+    TO_GLOBAL T* out = gptr<T>(d->out);
     const int64_t rmask = (1ll << nrest) - 1;
-    int64_t hi_key = -1, hi_in = 0, hi_out = 0;
-    for (int64_t t = t0; t < t1; ++t) {
+    const int lmask = (1 << nlo) - 1;
+    // base pointers of tile t (the same for the whole block, kept in scalar registers)
+    auto bases = [&](int64_t t, const TO_GLOBAL T*& ip, TO_GLOBAL T*& op) {
         const int64_t b = t >> nrest;
         const int64_t r = t & rmask;
-        const int64_t key = t >> nlo;
-        if (key != hi_key) {
-            const int64_t rh = r >> nlo;
-            hi_in = b * d->inb + (pdep(rh, d->rest_in + nlo, nrest - nlo));
-            hi_out = b * d->outb + (pdep(rh, d->rest_out + nlo, nrest - nlo));
-            hi_key = key;
-        }
-        const int rl = (int)(r & ((1 << nlo) - 1));
-        const int64_t ibase = hi_in + lo_in[rl];
-        const int64_t obase = hi_out + lo_out[rl];
-        __syncthreads();
+        const int64_t rh = r >> nlo;
+        int64_t ib = b * d->inb + pdep(rh, d->rest_in + nlo, nrest - nlo) + lo_in[r & lmask];
+        int64_t ob = b * d->outb + pdep(rh, d->rest_out + nlo, nrest - nlo) + lo_out[r & lmask];
+        ib = uniform64(ib);
+        ob = uniform64(ob);
+        ip = in + ib;
+        op = out + ob;
+    };
+    auto run = [&](auto full, auto off32) {
+        constexpr bool FULL = decltype(full)::value;
+        // in-tile offsets below 2^32 bytes address as 32-bit offsets from the scalar base
+        using Off = typename std::conditional<decltype(off32)::value, uint32_t, uint64_t>::type;
+        auto live = [&](int i) { return FULL || tid + kBlock * i < U; };
+        auto at = [](const TO_GLOBAL T* p, int e) {
+            return *reinterpret_cast<const TO_GLOBAL T*>(reinterpret_cast<const TO_GLOBAL char*>(p) +
+                                                        (Off)e * (Off)sizeof(T));
+        };
+        auto to = [](TO_GLOBAL T* p, int e) -> TO_GLOBAL T& {
+            return *reinterpret_cast<TO_GLOBAL T*>(reinterpret_cast<TO_GLOBAL char*>(p) + (Off)e * (Off)sizeof(T));
+        };
+        T val[kPermEpt];
+        if (t0 < t1) {
+            const TO_GLOBAL T* ip;
+            TO_GLOBAL T* op;
+            bases(t0, ip, op);
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i) {
-            const int u = threadIdx.x + kBlock * i;
-            if (u < U) tile[u] = in[ibase + in_off[i]];
+            for (int i = 0; i < kPermEpt; ++i)
+                if (live(i)) val[i] = at(ip, in_lo + hi_sum(bin, i));
         }
-        __syncthreads();
+        for (int64_t t = t0; t < t1; ++t) {
+            const TO_GLOBAL T* ip;
+            TO_GLOBAL T* op;
+            bases(t, ip, op);
+            __syncthreads();
 #pragma unroll
-        for (int i = 0; i < MAXE; ++i) {
-            const int v = threadIdx.x + kBlock * i;
-            if (v < U) out[obase + out_off[i]] = tile[ulds[i]];
+            for (int i = 0; i < kPermEpt; ++i)
+                if (live(i)) tile[wa_lo ^ hi_xor(bwa, i)] = val[i];
+            __syncthreads();
+            if (t + 1 < t1) {
+                const TO_GLOBAL T* ip2;
+                TO_GLOBAL T* op2;
+                bases(t + 1, ip2, op2);
+#pragma unroll
+                for (int i = 0; i < kPermEpt; ++i)
+                    if (live(i)) val[i] = at(ip2, in_lo + hi_sum(bin, i));
+            }
+#pragma unroll
+            for (int i = 0; i < kPermEpt; ++i)
+                if (live(i)) to(op, out_lo + hi_sum(bout, i)) = tile[ra_lo ^ hi_xor(bra, i)];
+        }
+    };
+    // full tiles (every entry of every thread live) take the path without guards
+    const bool off32 = ((uint64_t)sizeof(T) << d->rank) <= (1ull << 32);
+    if (U == kBlock * kPermEpt) {
+        if (off32) run(std::true_type{}, std::true_type{});
+        else run(std::true_type{}, std::false_type{});
+    } else {
+        run(std::false_type{}, std::false_type{});
+    }
+}
+
+// Vectorised tiles: 16-byte chunks (the lowest input bits) are loaded and the lowest
+// output bits are stored as 16-byte chunks.  The tile has exactly kPermVecBits<T> bits
+// (32 KB), so each thread moves 8 chunks per tile; the LDS swizzle only permutes whole
+// chunks, folding tile bits that the storing lanes vary into the bank bits above the chunk.
+
+template <typename T>
+constexpr int kPermVecBits = sizeof(T) == 4 ? 13 : 12;
+
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
+template <typename T>
+__device__ __forceinline__ void permute_tiles_vec(const to_perm_desc* __restrict__ d, int local, int nblocks,
+                                                  unsigned char* smem) {
+    constexpr int VL = 16 / sizeof(T);            // entries per chunk
+    constexpr int CB = sizeof(T) == 4 ? 2 : 1;    // log2(VL)
+    constexpr int NU = kPermVecBits<T>;
+    constexpr int NB = sizeof(T) == 4 ? 6 : 5;    // entry-address bits that select the bank
+    constexpr int NCH = (1 << NU) / VL / kBlock;  // chunks per thread per tile (8)
+    typedef typename Vec16<T>::type V;
+    T* tile = reinterpret_cast<T*>(smem);
+    int64_t* lo_in = reinterpret_cast<int64_t*>(smem + (sizeof(T) << NU));
+    int64_t* lo_out = lo_in + 256;
+    const int nrest = d->nrest;
+    const int nlo = nrest < 8 ? nrest : 8;
+    for (int r = threadIdx.x; r < (1 << nlo); r += kBlock) {
+        lo_in[r] = pdep(r, d->rest_in, nlo);
+        lo_out[r] = pdep(r, d->rest_out, nlo);
+    }
+    // swizzle: storing lanes vary v bits CB..CB+5, i.e. tile bits v2u[CB..CB+5]; those
+    // above the bank bits are xor-folded into free bank bits at or above CB
+    int zsrc[6], zdst[6], zn = 0;
+    {
+        int used = 0;
+        for (int j = CB; j < CB + 6; ++j)
+            if (d->v2u[j] < NB) used |= 1 << d->v2u[j];
+        int pos = CB;
+        for (int j = CB; j < CB + 6; ++j) {
+            const int b = d->v2u[j];
+            if (b < NB) continue;
+            while (pos < NB && ((used >> pos) & 1)) ++pos;
+            if (pos >= NB) break;
+            zsrc[zn] = b;
+            zdst[zn] = pos;
+            used |= 1 << pos;
+            ++zn;
         }
     }
+    auto swz = [&](int u) {
+        int a = u;
+        for (int i = 0; i < zn; ++i) a ^= ((u >> zsrc[i]) & 1) << zdst[i];
+        return a;
+    };
+    auto umap = [&](int e) {
+        int u = 0;
+        for (int j = 0; j < NU; ++j) u |= ((e >> j) & 1) << d->v2u[j];
+        return u;
+    };
+    // chunk q = tid + 256 i covers entries q*VL + c; every address below is linear in the
+    // bits of (tid, i, c), so it splits into a per-thread part and parts shared by the block
+    const int tid = threadIdx.x;
+    const int in_lo = (int)pdep(tid * VL, d->u_in, NU), out_lo = (int)pdep(tid * VL, d->v_out, NU);
+    const int wa_lo = swz(tid * VL), ra_lo = swz(umap(tid * VL));
+    int bin[3], bout[3], bwa[3], bra[3], rc[VL];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int e = (kBlock * VL) << k;
+        bin[k] = (int)pdep(e, d->u_in, NU);
+        bout[k] = (int)pdep(e, d->v_out, NU);
+        bwa[k] = swz(e);
+        bra[k] = swz(umap(e));
+    }
+#pragma unroll
+    for (int c = 0; c < VL; ++c) rc[c] = swz(umap(c));
+    auto hsum = [](const int* b, int i) {
+        return ((i & 1) ? b[0] : 0) + ((i & 2) ? b[1] : 0) + ((i & 4) ? b[2] : 0);
+    };
+    auto hxor = [](const int* b, int i) {
+        return ((i & 1) ? b[0] : 0) ^ ((i & 2) ? b[1] : 0) ^ ((i & 4) ? b[2] : 0);
+    };
+    __syncthreads();
+    const int64_t ntiles = (int64_t)d->nbatch << nrest;
+    const int64_t t0 = ntiles * local / nblocks, t1 = ntiles * (local + 1) / nblocks;
+    const TO_GLOBAL T* in = gptr<const T>(d->in);
+    TO_GLOBAL T* out = gptr<T>(d->out);
+    const int64_t rmask = (1ll << nrest) - 1;
+    const int lmask = (1 << nlo) - 1;
+    auto bases = [&](int64_t t, const TO_GLOBAL T*& ip, TO_GLOBAL T*& op) {
+        const int64_t b = t >> nrest;
+        const int64_t r = t & rmask;
+        const int64_t rh = r >> nlo;
+        int64_t ib = b * d->inb + pdep(rh, d->rest_in + nlo, nrest - nlo) + lo_in[r & lmask];
+        int64_t ob = b * d->outb + pdep(rh, d->rest_out + nlo, nrest - nlo) + lo_out[r & lmask];
+        ip = in + uniform64(ib);
+        op = out + uniform64(ob);
+    };
+    V val[NCH];
+    if (t0 < t1) {
+        const TO_GLOBAL T* ip;
+        TO_GLOBAL T* op;
+        bases(t0, ip, op);
+#pragma unroll
+        for (int i = 0; i < NCH; ++i) val[i] = *reinterpret_cast<const TO_GLOBAL V*>(ip + (in_lo + hsum(bin, i)));
+    }
+    for (int64_t t = t0; t < t1; ++t) {
+        const TO_GLOBAL T* ip;
+        TO_GLOBAL T* op;
+        bases(t, ip, op);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NCH; ++i) *reinterpret_cast<V*>(tile + (wa_lo ^ hxor(bwa, i))) = val[i];
+        __syncthreads();
+        if (t + 1 < t1) {
+            const TO_GLOBAL T* ip2;
+            TO_GLOBAL T* op2;
+            bases(t + 1, ip2, op2);
+#pragma unroll
+            for (int i = 0; i < NCH; ++i)
+                val[i] = *reinterpret_cast<const TO_GLOBAL V*>(ip2 + (in_lo + hsum(bin, i)));
+        }
+#pragma unroll
+        for (int i = 0; i < NCH; ++i) {
+            const int ra = ra_lo ^ hxor(bra, i);
+            V o;
+#pragma unroll
+            for (int c = 0; c < VL; ++c) o[c] = tile[ra ^ rc[c]];
+            *reinterpret_cast<TO_GLOBAL V*>(op + (out_lo + hsum(bout, i))) = o;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute_vec_desc(const to_perm_desc* __restrict__ descs, int ndesc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const to_perm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    permute_tiles_vec<T>(d, blockIdx.x - d->block0, d->nblocks, smem);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute_vec(const to_perm_desc desc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    permute_tiles_vec<T>(&desc, blockIdx.x, desc.nblocks, smem);
+}
+
+// descriptor-driven bit permutation, used to bring the two operands of a dense join into
+// row-major [free][shared] form for the MFMA product
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute_desc(const to_perm_desc* __restrict__ descs, int ndesc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const to_perm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    permute_tiles<T>(d, blockIdx.x - d->block0, d->nblocks, smem);
+}
+
+// the same permutation with its descriptor passed by value (to_permute)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute(const to_perm_desc desc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    permute_tiles<T>(&desc, blockIdx.x, desc.nblocks, smem);
 }
 
 // ---------------------------------------------------------------------------------
@@ -721,8 +959,8 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
         const int64_t batch = t / per_batch;
         const int64_t rem = t - batch * per_batch;
         const int64_t tm = rem / tiles_n, tn = rem - (rem / tiles_n) * tiles_n;
-        const T* A = reinterpret_cast<const T*>(d->a) + batch * d->ab + (tm * BM + lrow) * K + lk;
-        const T* B = reinterpret_cast<const T*>(d->b) + batch * d->bb + (tn * BN + lrow) * K + lk;
+        const TO_GLOBAL T* A = gptr<const T>(d->a) + batch * d->ab + (tm * BM + lrow) * K + lk;
+        const TO_GLOBAL T* B = gptr<const T>(d->b) + batch * d->bb + (tn * BN + lrow) * K + lk;
         T sa = T(1), sb = T(1);
         int64_t ea = 0, eb = 0;
         if (SCALED) {
@@ -770,7 +1008,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
             }
             if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 2>(acc, wide);
         }
-        T* C = reinterpret_cast<T*>(d->c) + batch * d->cb;
+        TO_GLOBAL T* C = gptr<T>(d->c) + batch * d->cb;
         float mx = 0.0f;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
@@ -826,8 +1064,8 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
         const int64_t batch = t / per_batch;
         const int64_t rem = t - batch * per_batch;
         const int64_t tm = rem / tiles_n, tn = rem - (rem / tiles_n) * tiles_n;
-        const T* A = reinterpret_cast<const T*>(d->a) + batch * d->ab + tm * BM * K;
-        const T* B = reinterpret_cast<const T*>(d->b) + batch * d->bb + tn * BN * K;
+        const TO_GLOBAL T* A = gptr<const T>(d->a) + batch * d->ab + tm * BM * K;
+        const TO_GLOBAL T* B = gptr<const T>(d->b) + batch * d->bb + tn * BN * K;
         T sa = T(1), sb = T(1);
         int64_t ea = 0, eb = 0;
         if (SCALED) {
@@ -849,8 +1087,8 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
         L ra[NV], rb[NV];
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
-            ra[i] = *reinterpret_cast<const L*>(A + vrow[i] * K + vk[i]);
-            rb[i] = *reinterpret_cast<const L*>(B + vrow[i] * K + vk[i]);
+            ra[i] = *reinterpret_cast<const TO_GLOBAL L*>(A + vrow[i] * K + vk[i]);
+            rb[i] = *reinterpret_cast<const TO_GLOBAL L*>(B + vrow[i] * K + vk[i]);
         }
         for (int64_t k0 = 0; k0 < K; k0 += BK, ++step) {
             __syncthreads();
@@ -866,8 +1104,8 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
             if (k0 + BK < K) {
 #pragma unroll
                 for (int i = 0; i < NV; ++i) {
-                    ra[i] = *reinterpret_cast<const L*>(A + vrow[i] * K + k0 + BK + vk[i]);
-                    rb[i] = *reinterpret_cast<const L*>(B + vrow[i] * K + k0 + BK + vk[i]);
+                    ra[i] = *reinterpret_cast<const TO_GLOBAL L*>(A + vrow[i] * K + k0 + BK + vk[i]);
+                    rb[i] = *reinterpret_cast<const TO_GLOBAL L*>(B + vrow[i] * K + k0 + BK + vk[i]);
                 }
             }
 #pragma unroll
@@ -886,7 +1124,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
             if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 4>(acc, wide);
         }
 
-        T* C = reinterpret_cast<T*>(d->c) + batch * d->cb;
+        TO_GLOBAL T* C = gptr<T>(d->c) + batch * d->cb;
         float mx = 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -979,8 +1217,12 @@ int launch_one(const to_launch& L, hipStream_t st) {
             break;
         case TO_OP_PERMUTE:
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
-            hipLaunchKernelGGL((k_permute_desc<T>), grid, block, L.lds, st,
-                               reinterpret_cast<const to_perm_desc*>(L.descs), L.count);
+            if (L.variant == 1)
+                hipLaunchKernelGGL((k_permute_vec_desc<T>), grid, block, L.lds, st,
+                                   reinterpret_cast<const to_perm_desc*>(L.descs), L.count);
+            else
+                hipLaunchKernelGGL((k_permute_desc<T>), grid, block, L.lds, st,
+                                   reinterpret_cast<const to_perm_desc*>(L.descs), L.count);
             break;
         case TO_OP_GEMM:
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
@@ -1170,7 +1412,8 @@ int to_graph_destroy(void* graph_exec) {
 int to_permute(const void* in, void* out, int rank, const int32_t* perm, int elem_bytes, int64_t batch,
                void* stream) {
     g_err[0] = 0;
-    if (!in || !out || !perm || rank < 0 || rank > 31 || batch <= 0 || (elem_bytes != 4 && elem_bytes != 8)) {
+    if (!in || !out || !perm || rank < 0 || rank > 31 || batch <= 0 || batch > INT32_MAX ||
+        (elem_bytes != 4 && elem_bytes != 8)) {
         set_err("to_permute: invalid argument");
         return TO_ERR_INVALID;
     }
@@ -1187,7 +1430,11 @@ int to_permute(const void* in, void* out, int rank, const int32_t* perm, int ele
     }
     int ipos_of_out[32];
     for (int p = 0; p < rank; ++p) ipos_of_out[opos[p]] = p;
-    // grow m while the tile union stays <= 2^10 entries
+    // grow m while the tile union stays within the tile bits; a tensor of at least that
+    // many bits gets exactly that many (extra low input bits), which the vectorised
+    // kernel needs
+    const bool aligned = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    const int tile_bits = (aligned && elem_bytes == 4) ? 13 : 12;  // the scalar kernel takes <= 12
     int m = 0, inU[32] = {0};
     for (;;) {
         if (m >= rank) break;
@@ -1197,13 +1444,26 @@ int to_permute(const void* in, void* out, int rank, const int32_t* perm, int ele
         cand[ipos_of_out[m]] = 1;
         int cnt = 0;
         for (int p = 0; p < rank; ++p) cnt += cand[p];
-        if (cnt > 10) break;
+        if (cnt > tile_bits) break;
         memcpy(inU, cand, sizeof(cand));
         ++m;
     }
-    PermPlan p;
+    {
+        int cnt = 0;
+        for (int q = 0; q < rank; ++q) cnt += inU[q];
+        for (int q = 0; q < rank && cnt < tile_bits; ++q)
+            if (!inU[q]) {
+                inU[q] = 1;
+                ++cnt;
+            }
+    }
+    to_perm_desc p;
     memset(&p, 0, sizeof(p));
+    p.in = reinterpret_cast<uint64_t>(in);
+    p.out = reinterpret_cast<uint64_t>(out);
+    p.inb = p.outb = 1ll << rank;
     p.rank = rank;
+    p.nbatch = (int32_t)batch;
     int nu = 0;
     for (int q = 0; q < rank; ++q)
         if (inU[q]) p.u_in[nu++] = (int8_t)q;
@@ -1228,15 +1488,21 @@ int to_permute(const void* in, void* out, int rank, const int32_t* perm, int ele
     }
     p.nrest = nr;
     const int64_t ntiles = batch << nr;
-    const int grid = (int)(ntiles < 8192 ? ntiles : 8192);
-    const size_t lds = (size_t)elem_bytes << nu;
+    const bool vec = aligned && nu == tile_bits;
+    const int64_t cap = vec ? 1024 : 4096;
+    const int grid = (int)(ntiles < cap ? ntiles : cap);
+    p.block0 = 0;
+    p.nblocks = grid;
+    const size_t lds = (((size_t)elem_bytes << nu) + 15) / 16 * 16 + 2 * 256 * sizeof(int64_t);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (elem_bytes == 8)
-        hipLaunchKernelGGL(k_permute<double>, dim3(grid), dim3(kBlock), lds, st, (const double*)in, (double*)out,
-                           p, batch);
+    if (vec && elem_bytes == 8)
+        hipLaunchKernelGGL(k_permute_vec<double>, dim3(grid), dim3(kBlock), lds, st, p);
+    else if (vec)
+        hipLaunchKernelGGL(k_permute_vec<float>, dim3(grid), dim3(kBlock), lds, st, p);
+    else if (elem_bytes == 8)
+        hipLaunchKernelGGL(k_permute<double>, dim3(grid), dim3(kBlock), lds, st, p);
     else
-        hipLaunchKernelGGL(k_permute<float>, dim3(grid), dim3(kBlock), lds, st, (const float*)in, (float*)out, p,
-                           batch);
+        hipLaunchKernelGGL(k_permute<float>, dim3(grid), dim3(kBlock), lds, st, p);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? TO_OK : hip_fail(e, "to_permute launch");
 }

@@ -467,11 +467,11 @@ class HipAPI(BaseTensorAPI):
                 for i, p in enumerate(fpos):
                     out_pos[p] = len(spos) + i
                 dst = self._alloc((1 << T.rank) * self._esize)
-                pd, grid, lds = schedule.perm_desc(T.data.data_ptr(), dst.data_ptr(), T.rank, out_pos, 1,
-                                                   1 << T.rank, self._esize)
+                pd, grid, lds, pvar = schedule.perm_desc(T.data.data_ptr(), dst.data_ptr(), T.rank, out_pos, 1,
+                                                         1 << T.rank, self._esize)
                 pblob = self._alloc(ctypes.sizeof(pd))
                 self._upload(pblob, bytes(pd))
-                launches.append(_lib.Launch(op=_lib.TO_OP_PERMUTE, count=1, grid=grid, lds=lds,
+                launches.append(_lib.Launch(op=_lib.TO_OP_PERMUTE, variant=pvar, count=1, grid=grid, lds=lds,
                                             descs=pblob.data_ptr()))
                 keep += [dst, pblob]
                 ops.append(dst.data_ptr())

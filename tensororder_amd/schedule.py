@@ -305,10 +305,11 @@ class Program:
         self.stats = {}
 
 
-def perm_plan(rank, out_pos, max_tile_bits=10):
+def perm_plan(rank, out_pos, max_tile_bits=12):
     """LDS tile of a bit permutation (input bit p -> output bit out_pos[p]): the union U of
     the m lowest input bits and the input bits landing on the m lowest output bits, m as
-    large as keeps |U| <= max_tile_bits."""
+    large as keeps |U| <= max_tile_bits; a tensor of at least max_tile_bits bits gets
+    exactly that many (the lowest input bits not yet in U are added)."""
     ipos_of_out = [0] * rank
     for p, q in enumerate(out_pos):
         ipos_of_out[q] = p
@@ -320,6 +321,10 @@ def perm_plan(rank, out_pos, max_tile_bits=10):
             break
         U = cand
         m += 1
+    for p in range(rank):
+        if len(U) >= max_tile_bits:
+            break
+        U.add(p)
     u_in = sorted(U)
     v_out, v2u = [], []
     for q in range(rank):
@@ -331,9 +336,16 @@ def perm_plan(rank, out_pos, max_tile_bits=10):
     return u_in, v_out, v2u, rest, [out_pos[p] for p in rest]
 
 
+# tile bits of the vectorised permute kernel (32 KB tiles of 16-byte chunks)
+PERM_VEC_BITS = {4: 13, 8: 12}
+
+
 def perm_desc(src, dst, rank, out_pos, nbatch, stride, esize):
-    """(to_perm_desc, grid, LDS bytes) of a batched bit permutation."""
-    u_in, v_out, v2u, rest_in, rest_out = perm_plan(rank, out_pos)
+    """(to_perm_desc, grid, LDS bytes, variant) of a batched bit permutation; variant 1 is
+    the vectorised kernel (tensors of at least its tile bits, 16-byte aligned buffers)."""
+    vec_bits = PERM_VEC_BITS[esize]
+    vec = rank >= vec_bits and src % 16 == 0 and dst % 16 == 0 and (stride * esize) % 16 == 0
+    u_in, v_out, v2u, rest_in, rest_out = perm_plan(rank, out_pos, vec_bits if vec else 12)
     d = _lib.PermDesc()
     d.in_, d.out = src, dst
     d.inb = d.outb = stride
@@ -343,9 +355,11 @@ def perm_desc(src, dst, rank, out_pos, nbatch, stride, esize):
             arr[i] = x
     ntiles = nbatch << len(rest_in)
     d.block0 = 0
-    d.nblocks = int(min(ntiles, 8192))
+    # the chunk kernel holds 4 blocks per CU (36 KB of LDS each): one resident wave of
+    # blocks, each walking a contiguous range of tiles
+    d.nblocks = int(min(ntiles, 1024 if vec else 4096))
     lds = ((esize << len(u_in)) + 15) // 16 * 16 + 2 * 256 * 8
-    return d, d.nblocks, lds
+    return d, d.nblocks, lds, 1 if vec else 0
 
 
 def _pos_array(vals):
@@ -455,8 +469,8 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
                     out_pos[p] = t
                 for i, p in enumerate(fpos):
                     out_pos[p] = len(spos) + i
-                pd, grid, lds = perm_desc(op.ptr, dst, op.rank, out_pos, nb_op, 1 << op.rank, esize)
-                out.append(("permute", _lib.TO_OP_PERMUTE, 0, lds, pd, grid))
+                pd, grid, lds, pvar = perm_desc(op.ptr, dst, op.rank, out_pos, nb_op, 1 << op.rank, esize)
+                out.append(("permute", _lib.TO_OP_PERMUTE, pvar, lds, pd, grid))
                 operands.append((dst, stride, op))
             else:
                 operands.append((op.ptr, stride, op))
@@ -615,13 +629,20 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
         for kind, op, variant, lds, st, nblk in join_records(n, nodes, W, esize, scaled, abase,
                                                                partial.get(n.idx, {}), recp):
             off = put([st])
-            phase_list.append((kind, Launch(op=op, variant=variant, count=1, grid=nblk, lds=lds), off, n.idx))
+            if kind == "permute":
+                work = (2 * esize * (int(st.nbatch) << int(st.rank)), 0)
+            elif kind in ("stream", "splitk", "gemm"):
+                work = join_traffic(n, nodes, W, esize)
+            else:
+                work = (0, 0)
+            phase_list.append((kind, Launch(op=op, variant=variant, count=1, grid=nblk, lds=lds), off, n.idx,
+                               work))
 
     pro, wave = [], []
     if scaled and prog.rec_ranges[1] > prog.rec_ranges[0]:
         a, b = prog.rec_ranges[0], prog.rec_ranges[1]
         pro.append(("clear", Launch(op=_lib.TO_OP_SCALE_CLEAR, grid=max(1, min(1024, -(-(b - a) // 256))),
-                                    aux0=rbase + a * rec_size, iaux0=b - a), None, None))
+                                    aux0=rbase + a * rec_size, iaux0=b - a), None, None, (0, 0)))
     for n in nodes:
         if not n.is_leaf and not n.batched:
             join_launches(n, pro)
@@ -630,7 +651,7 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
     if scaled and prog.rec_ranges[2] > prog.rec_ranges[1]:
         a, b = prog.rec_ranges[1], prog.rec_ranges[2]
         wave.append(("clear", Launch(op=_lib.TO_OP_SCALE_CLEAR, grid=max(1, min(1024, -(-(b - a) // 256))),
-                                     aux0=rbase + a * rec_size, iaux0=b - a), None, None))
+                                     aux0=rbase + a * rec_size, iaux0=b - a), None, None, (0, 0)))
     leaf_descs = []
     block0 = 0
     for n in nodes:
@@ -650,8 +671,9 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
             leaf_descs.append(ld)
     if leaf_descs:
         off = put(leaf_descs)
+        leaf_bytes = sum(esize * (W << ld.rank) for ld in leaf_descs)
         wave.append(("leaf", Launch(op=_lib.TO_OP_LEAF_GATHER, count=len(leaf_descs), grid=block0,
-                                    aux3=wave_params), off, None))
+                                    aux3=wave_params), off, None, (leaf_bytes, 0)))
     for n in nodes:
         if not n.is_leaf and n.batched:
             join_launches(n, wave)
@@ -660,21 +682,27 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
                          aux3=pro_params if prog.root_const else wave_params,
                          iaux0=1 if prog.root_const else W)
     if prog.root_const:
-        pro.append(("root", root_launch, None, None))
+        pro.append(("root", root_launch, None, None, (0, 0)))
     else:
-        wave.append(("root", root_launch, None, None))
-        wave.append(("advance", Launch(op=_lib.TO_OP_WAVE_ADVANCE, grid=1, aux3=wave_params, iaux0=W), None, None))
+        wave.append(("root", root_launch, None, None, (0, 0)))
+        wave.append(("advance", Launch(op=_lib.TO_OP_WAVE_ADVANCE, grid=1, aux3=wave_params, iaux0=W), None, None,
+                     (0, 0)))
 
     prog.blob_host = np.frombuffer(bytes(blob) or b"\0", dtype=np.uint8).copy()
     prog.blob = device_alloc(prog.blob_host.nbytes)
     bbase = prog.blob.data_ptr()
+    # per launch: (kind, node index) and its algorithmic (bytes, flops): each distinct
+    # operand entry read once and each output entry written once
     prog.prologue_info, prog.wave_info = [], []
-    for lst, out, info in ((pro, prog.prologue, prog.prologue_info), (wave, prog.wave, prog.wave_info)):
-        for kind, L, off, nidx in lst:
+    prog.prologue_work, prog.wave_work = [], []
+    for lst, out, info, work in ((pro, prog.prologue, prog.prologue_info, prog.prologue_work),
+                                 (wave, prog.wave, prog.wave_info, prog.wave_work)):
+        for kind, L, off, nidx, w in lst:
             if off is not None:
                 L.descs = bbase + off
             out.append(L)
             info.append((kind, nidx))
+            work.append(w)
     prog.stats = dict(
         nodes=len(nodes),
         joins=sum(1 for n in nodes if not n.is_leaf),

@@ -164,13 +164,17 @@ def test_tensordot_large_shared_splitk(hip):
     np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-11)
 
 
-def test_permute_matches_numpy(hip):
+def test_permute_matches_numpy(hip, hip32):
+    # ranks below and above the tile sizes: scalar tiles (<= 12 bits) and the 16-byte
+    # chunk kernel (tiles of 13 bits for float, 12 for double)
     rng = np.random.default_rng(3)
     _say("case permute")
-    for r in (0, 1, 3, 7, 12, 16):
-        a = rng.standard_normal((2,) * r)
-        perm = [int(x) for x in rng.permutation(r)]
-        np.testing.assert_array_equal(hip.permute(a, perm), np.transpose(a, perm))
+    for api, dt in ((hip, np.float64), (hip32, np.float32)):
+        for r in (0, 1, 3, 7, 12, 13, 14, 16, 18):
+            a = rng.standard_normal((2,) * r).astype(dt)
+            for perm in ([int(x) for x in rng.permutation(r)], list(range(r))[::-1]):
+                got = np.asarray(api.permute(a, perm), dtype=dt)
+                np.testing.assert_array_equal(got, np.transpose(a, perm))
 
 
 def test_execution_run_slices_on_oom(hip):
