@@ -14,6 +14,7 @@ every candidate is an exact plan of the same network.
 
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -25,6 +26,10 @@ FEATURES = [
     "variant",
 ]
 TARGETS = ["log_time_us", "log_footprint_bytes"]
+
+# parameters trained by ``python -m tensororder_amd.train_costmodel`` on the executor's own
+# timed records (see the summary json beside them)
+DEFAULT_PARAMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "costmodel_params.npz")
 
 
 def join_features(n, nodes, W, esize):
@@ -72,6 +77,22 @@ def records_from_timing(prog, infos, ms, W, esize):
         X.append(join_features(n, prog.nodes, W, esize))
         Y.append([math.log(max(t, 1e-6) * 1e3), math.log(join_footprint(n, prog.nodes, W, esize))])
     return np.asarray(X, dtype=np.float32), np.asarray(Y, dtype=np.float32)
+
+
+def plan_flops(nodes, tables, esize, entry_type):
+    """(flops per slice of the batched joins, flops of the slice-invariant joins) of an
+    analysed plan."""
+    schedule.peak_bytes(nodes, tables, 1, entry_type)
+    per_slice = const = 0.0
+    for n in nodes:
+        if n.is_leaf:
+            continue
+        _, f = schedule.join_traffic(n, nodes, 1, esize)
+        if n.batched:
+            per_slice += f
+        else:
+            const += f
+    return per_slice, const
 
 
 def plan_launch_features(nodes, W, esize, nwaves):
@@ -229,6 +250,21 @@ def reference_choice(plans):
 
 
 def learned_choice(model, api, plans):
-    """Index of the candidate with the smallest predicted device time, and all predictions."""
-    preds = [predicted_seconds(model, api, p) for p in plans]
+    """Index of the candidate with the smallest predicted device time, and all predictions.
+    A candidate that does not fit the device even one slice at a time is never chosen."""
+    from .api import OutOfMemoryError
+
+    preds = []
+    for p in plans:
+        try:
+            preds.append(predicted_seconds(model, api, p))
+        except OutOfMemoryError:
+            preds.append(math.inf)
     return int(np.argmin(preds)), preds
+
+
+def load_default(api):
+    """The trained model of ``DEFAULT_PARAMS`` on ``api``'s device."""
+    model = CostModel(api)
+    model.load(DEFAULT_PARAMS)
+    return model

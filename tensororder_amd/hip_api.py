@@ -287,8 +287,10 @@ class HipAPI(BaseTensorAPI):
         _lib.check(self._lib.to_synchronize(ctypes.c_void_p(self._stream.cuda_stream)), "to_synchronize")
 
     def _budget(self):
+        # free device memory plus what torch's caching allocator holds without using it
         free, _total = self._torch.cuda.mem_get_info(self._device)
-        budget = int(free * 0.85)
+        cached = self._torch.cuda.memory_reserved(self._device) - self._torch.cuda.memory_allocated(self._device)
+        budget = int((free + max(0, cached)) * 0.85)
         if self._mem_limit is not None:
             budget = min(budget, self._mem_limit)
         return budget

@@ -217,3 +217,66 @@ def best_plan(network, planner, solver=None, max_decompositions=1, seed=None):
         if best is None or best.tree.maxrank > tree.maxrank:
             best = SlicedExecutionPlan(tree, net)
     return best
+
+
+# Slice sets tried per decomposition when choosing a plan by predicted cost: the reference's
+# greedy_mem (``slicers.py:27-33``), greedy_most at rank caps (``slicers.py:52-99``) and
+# greedy_largest under fixed random seeds (``slicers.py:36-49``); (slicer, slice_until
+# keyword arguments, random seed).
+SLICE_SPECS = (
+    ("greedy_mem", {}, 0),
+    ("greedy_most", {"rank": 25}, 0),
+    ("greedy_most", {"rank": 26}, 0),
+    ("greedy_most", {"rank": 27}, 0),
+    ("greedy_most", {"rank": 28}, 0),
+    ("greedy_largest", {}, 0),
+    ("greedy_largest", {}, 1),
+    ("greedy_largest", {}, 2),
+)
+
+
+def plan_candidates(network, planner, decompositions, slices, specs=SLICE_SPECS, seed=0):
+    """Candidate plans of one network: every decomposition (TreeDecomposition) x every
+    slice spec, each sliced to exactly ``slices`` groups (a spec that needs more groups to
+    meet its own limits is dropped).  Returns (plans, labels); a label names the
+    decomposition index, its unsliced max rank and the slice spec."""
+    import random as _random
+
+    from .plan import ALL_SLICERS, SlicedExecutionPlan
+
+    plans, labels = [], []
+    for di, td in enumerate(decompositions):
+        width = td.width()
+        for slicer, kw, rseed in specs:
+            if planner == "line":
+                tree, net = plan_line(network, td)
+            else:
+                tree, net = plan_factor(network, td, seed=seed)
+            tree.treewidth = width
+            plan = SlicedExecutionPlan(tree, net)
+            unsliced = plan.maxrank
+            state = _random.getstate()
+            _random.seed(rseed)
+            try:
+                ALL_SLICERS[slicer].slice_until(plan, slices=slices, **kw)
+            finally:
+                _random.setstate(state)
+            if len(plan.groups_to_slice) != slices:
+                continue
+            plans.append(plan)
+            labels.append({"decomposition": di, "width": width, "unsliced_maxrank": unsliced,
+                           "slicer": slicer, "args": dict(kw), "seed": rseed, "maxrank": plan.maxrank})
+    return plans, labels
+
+
+def reference_candidate(labels):
+    """The reference's choice among candidates: the decomposition whose plan has the
+    smallest unsliced max rank (first on ties, ``planning.py:159-162``), sliced by
+    greedy_mem."""
+    best = None
+    for i, lab in enumerate(labels):
+        if lab["slicer"] != "greedy_mem":
+            continue
+        if best is None or lab["unsliced_maxrank"] < labels[best]["unsliced_maxrank"]:
+            best = i
+    return best

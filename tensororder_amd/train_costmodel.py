@@ -1,18 +1,21 @@
 """Collect per-join timing records with the executor, train the cost MLP on the GPU, and
-compare its plan choice with the reference rule on held-out instances.
+compare its plan choice with the reference rule on a held-out candidate set.
 
     python -m tensororder_amd.train_costmodel --out DIR [--instances N] [--epochs E]
+        [--candidates tests/golden/cnf70_candidates.json] [--measure]
 
-Training instances: seeded random 3-CNFs (variables and clause ratio drawn per instance)
-planned with the factor-tree planner over the built-in min-degree decomposition and sliced
-by greedy_mem to a rank cap; vertex-cover instances with the line planner.  For each, one
-wave of slices is run with per-launch HIP event timing (``to_run_timed``) in both entry
-types; every join contributes one record (features -> log time, log footprint).
+Training instances: seeded random 3-CNFs (56-100 variables, clause ratio 1.6-2.2) planned
+with the factor-tree planner over the built-in min-degree decomposition (random ties) and
+sliced by a random slice spec of ``planners.SLICE_SPECS``; every fourth instance is a
+weighted vertex cover planned with the line planner.  For each, one wave of slices (its
+size capped so that a wave stays short) is run with per-launch HIP event timing
+(``to_run_timed``) in both entry types; every join contributes one record
+(16 features -> log device time, log memory footprint).
 
-Held-out evaluation: for each held-out formula, candidate plans = 4 decompositions x the
-slice sets {greedy_mem, greedy_most} x rank caps; the reference rule takes the smallest
-unsliced max rank with greedy_mem at the first cap; the learned rule takes the smallest
-predicted time.  Both choices are run in full and timed; their counts must agree.
+Held-out evaluation: the candidate fixture (decompositions x slice specs of one formula);
+the reference rule takes the decomposition of smallest unsliced max rank with greedy_mem,
+the learned rule the smallest predicted time.  With ``--measure`` both choices are
+contracted in full with the FP32-scaled executor and timed; their counts must agree.
 """
 
 import argparse
@@ -20,28 +23,36 @@ import ctypes
 import json
 import math
 import os
-import sys
 import time
 
 import numpy as np
 
-from . import _lib, costmodel, network, planners, schedule
+from . import _lib, costmodel, decomposition, network, planners, schedule
 from .formula import parse_dimacs_text
 from .instances import random_3cnf, vertex_cover_cnf
 from .plan import ALL_SLICERS
+
+WAVE_FLOPS = 2e13  # one timed wave of a training plan stays below this many flops
 
 
 def log(*a):
     print(*a, flush=True)
 
 
-def timed_records(api, plan, max_waves=1):
-    """Records of one contraction limited to ``max_waves`` waves."""
+def timed_records(api, plan, wave_flops=WAVE_FLOPS):
+    """Records of the first wave of one contraction, the wave capped at ``wave_flops``."""
     total, tree, tables, groups = api._plan_inputs(plan, None)
     nodes = schedule.analyse(tree, tables, api._regroup)
-    W = api.choose_batch(nodes, tables, total)
-    limit = min(total, W * max_waves)
-    exe = api._compile(tree, tables, groups, limit)
+    esize = api.get_entry_size()
+    per_slice, _ = costmodel.plan_flops(nodes, tables, esize, api._entry_type)
+    cap = max(1, int(wave_flops // max(per_slice, 1.0)))
+    saved = api._max_batch
+    api._max_batch = max(1, min(saved, 1 << int(math.log2(cap))))
+    try:
+        W = api.choose_batch(nodes, tables, total)
+        exe = api._compile(tree, tables, groups, min(total, W))
+    finally:
+        api._max_batch = saved
     try:
         launches = list(exe.step_launches)
         info = [("begin", None)] + list(exe.prog.prologue_info)
@@ -52,72 +63,55 @@ def timed_records(api, plan, max_waves=1):
         ms = (ctypes.c_float * len(launches))()
         _lib.check(api._lib.to_run_timed(ctypes.addressof(arr), len(launches), api._entry_type,
                                          ctypes.c_void_p(api._stream.cuda_stream), ms), "to_run_timed")
-        # the prologue runs once; wave joins appear once per wave -- keep the first wave
-        seen, keep_info, keep_ms = set(), [], []
-        for (kind, nidx), t in zip(info, ms):
-            key = (kind, nidx)
-            if key in seen:
-                continue
-            seen.add(key)
-            keep_info.append((kind, nidx))
-            keep_ms.append(t)
-        return costmodel.records_from_timing(exe.prog, keep_info, keep_ms, exe.prog.nbatch,
-                                             api.get_entry_size())
+        return costmodel.records_from_timing(exe.prog, info, list(ms), exe.prog.nbatch, esize)
     finally:
         exe.close()
 
 
-def training_plans(count, seed0=0, rank_cap=22):
+def training_plans(count, seed0=0):
+    """(index, plan) of the training corpus (deterministic for a seed)."""
+    import random
+
     rng = np.random.default_rng(seed0)
     for i in range(count):
         if i % 4 == 3:
-            n = int(rng.integers(20, 60)) * 2
+            n = int(rng.integers(30, 101)) * 2
             f = parse_dimacs_text(vertex_cover_cnf(n, seed0 + i, True), "cachet")
-            plan = planners.best_plan(network.cnf_count(f), "line", solver=None)
+            plan = planners.candidate_plans(network.cnf_count(f), "line", [int(rng.integers(1 << 30))])[0]
+            cap = int(rng.integers(16, 25))
+            random.seed(seed0 + i)
+            ALL_SLICERS["greedy_mem"].slice_until(plan, rank=cap)
         else:
-            n = int(rng.integers(30, 65))
+            n = int(rng.integers(56, 101))
             ratio = float(rng.uniform(1.6, 2.2))
             f = parse_dimacs_text(random_3cnf(n, int(round(ratio * n)), seed0 + i, True), "cachet")
             plan = planners.candidate_plans(network.cnf_count(f), "factor", [int(rng.integers(1 << 30))])[0]
-        ALL_SLICERS["greedy_mem"].slice_until(plan, rank=rank_cap)
+            slicer, kw, rseed = planners.SLICE_SPECS[int(rng.integers(len(planners.SLICE_SPECS)))]
+            groups = int(rng.integers(6, 15))
+            random.seed(rseed + 1000 * i)
+            ALL_SLICERS[slicer].slice_until(plan, slices=groups, **kw)
+            if plan.maxrank > 27:
+                ALL_SLICERS["greedy_mem"].slice_until(plan, rank=27)
         yield i, plan
 
 
-def candidates(formula_text, tie_seeds=(0, 1, 2, 3), caps=(20, 21, 22, 23)):
-    """(plans, labels) for 4 decompositions x {greedy_mem, greedy_most} x rank caps."""
-    import random
-
-    f = parse_dimacs_text(formula_text, "cachet")
-    net = network.cnf_count(f)
-    plans, labels = [], []
-    for ts in tie_seeds:
-        for slicer in ("greedy_mem", "greedy_most"):
-            for cap in caps:
-                base = planners.candidate_plans(net, "factor", [ts])[0]
-                random.seed(ts * 1000 + cap)
-                ALL_SLICERS[slicer].slice_until(base, rank=cap)
-                plans.append(base)
-                labels.append(dict(decomposition=ts, slicer=slicer, rank_cap=cap,
-                                   unsliced_rank=None, groups=len(base.groups_to_slice)))
-    return plans, labels
+def load_candidates(path):
+    """(plans, labels, reference index, DIMACS text) of a candidate fixture."""
+    with open(path) as f:
+        doc = json.load(f)
+    net = network.cnf_count(parse_dimacs_text(doc["dimacs"], doc["weights"]))
+    tds = [decomposition.parse_all(t["text"])[0] for t in doc["decompositions"]]
+    planner = "line" if doc["planner"].startswith("line") else "factor"
+    plans, labels = planners.plan_candidates(net, planner, tds, doc["slices"])
+    return plans, labels, planners.reference_candidate(labels), doc
 
 
-def reference_index(formula_text, labels, plans, tie_seeds=(0, 1, 2, 3), cap=20):
-    """The reference rule: the decomposition of smallest unsliced max rank, greedy_mem."""
-    f = parse_dimacs_text(formula_text, "cachet")
-    net = network.cnf_count(f)
-    ranks = [planners.candidate_plans(net, "factor", [ts])[0].maxrank for ts in tie_seeds]
-    best_ts = tie_seeds[int(np.argmin(ranks))]
-    for i, lab in enumerate(labels):
-        if lab["decomposition"] == best_ts and lab["slicer"] == "greedy_mem" and lab["rank_cap"] == cap:
-            return i
-    return 0
-
-
-def measure(api, plan, reps=2):
+def measure(api, plan, reps=1, warmup=1):
+    """(seconds per full contraction, count) with inputs resident."""
     exe = api.prepare(plan)
     try:
-        exe.run()
+        for _ in range(warmup):
+            exe.run()
         t0 = time.perf_counter()
         for _ in range(reps):
             res = exe.run()
@@ -129,10 +123,10 @@ def measure(api, plan, reps=2):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--instances", type=int, default=40)
+    ap.add_argument("--instances", type=int, default=200)
     ap.add_argument("--epochs", type=int, default=400)
-    ap.add_argument("--heldout", type=int, default=2)
-    ap.add_argument("--heldout-vars", type=int, default=56)
+    ap.add_argument("--candidates", default=None)
+    ap.add_argument("--measure", action="store_true")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
     from .hip_api import HipAPI, HipF32API
@@ -145,7 +139,7 @@ def main():
             x, y = timed_records(api, plan)
             X.append(x)
             Y.append(y)
-        log("records: instance %d maxrank %d groups %d total rows %d (%.0f s)"
+        log("records: instance %d maxrank %d groups %d rows %d (%.0f s)"
             % (i, plan.maxrank, len(plan.groups_to_slice), sum(len(a) for a in X), time.time() - t0))
     X = np.concatenate(X)
     Y = np.concatenate(Y)
@@ -157,24 +151,27 @@ def main():
     pred = model.predict(X)
     err = np.abs(pred[:, 0] - Y[:, 0])
     log("train |log time error| median %.3f p90 %.3f" % (np.median(err), np.percentile(err, 90)))
+    summary = dict(rows=int(len(X)), instances=args.instances, epochs=args.epochs, loss_first=hist[0],
+                   loss_last=hist[-1], train_log_time_err_median=float(np.median(err)),
+                   train_log_time_err_p90=float(np.percentile(err, 90)))
 
-    summary = dict(rows=int(len(X)), loss_first=hist[0], loss_last=hist[-1],
-                   train_log_time_err_median=float(np.median(err)), heldout=[])
-    api = apis["fp32s"]
-    for h in range(args.heldout):
-        text = random_3cnf(args.heldout_vars, 2 * args.heldout_vars, 10000 + h, True)
-        plans, labels = candidates(text)
-        ref = reference_index(text, labels, plans)
+    if args.candidates:
+        api = apis["fp32s"]
+        plans, labels, ref, doc = load_candidates(args.candidates)
         choice, preds = costmodel.learned_choice(model, api, plans)
-        t_ref, v_ref = measure(api, plans[ref])
-        t_new, v_new = measure(api, plans[choice])
-        rel = abs(v_new - v_ref) / abs(v_ref)
-        entry = dict(seed=10000 + h, reference=labels[ref], learned=labels[choice],
-                     reference_seconds=t_ref, learned_seconds=t_new, speedup=t_ref / t_new,
-                     predicted_reference=preds[ref], predicted_learned=preds[choice], count_rel_diff=rel)
-        summary["heldout"].append(entry)
-        log("held-out %d: reference %.4f s, learned %.4f s (x%.2f), counts rel diff %.2e"
-            % (h, t_ref, t_new, t_ref / t_new, rel))
+        summary["heldout"] = dict(name=doc["name"], candidates=len(plans), reference=ref, learned=choice,
+                                  reference_label=labels[ref], learned_label=labels[choice],
+                                  predicted_seconds=preds)
+        log("held-out %s: %d candidates, reference #%d predicted %.3f s, learned #%d predicted %.3f s"
+            % (doc["name"], len(plans), ref, preds[ref], choice, preds[choice]))
+        if args.measure:
+            t_ref, v_ref = measure(api, plans[ref])
+            t_new, v_new = measure(api, plans[choice])
+            rel = abs(v_new - v_ref) / abs(v_ref)
+            summary["heldout"].update(reference_seconds=t_ref, learned_seconds=t_new, speedup=t_ref / t_new,
+                                      count_rel_diff=rel)
+            log("measured: reference %.3f s, learned %.3f s (x%.2f), counts rel diff %.2e"
+                % (t_ref, t_new, t_ref / t_new, rel))
     with open(os.path.join(args.out, "costmodel_summary.json"), "w") as f:
         json.dump(summary, f, indent=1)
 

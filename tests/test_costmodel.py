@@ -2,6 +2,8 @@
 closed-form Adam step (CPU); the HIP forward/backward/Adam kernels match it and training
 reduces the loss (GPU)."""
 
+import os
+
 import numpy as np
 import pytest
 
@@ -82,3 +84,53 @@ def test_hip_mlp_matches_oracle_and_trains():
     # training reduces the loss
     hist = model.fit(x, t, epochs=60, batch=512, lr=3e-3)
     assert hist[-1] < 0.5 * hist[0]
+
+
+CANDIDATES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cnf70_candidates.json")
+
+
+def test_candidate_plans_share_one_count():
+    # every candidate (decomposition x slice spec) is an exact plan of the same network:
+    # the oracle's float64 counts agree, whichever one a rule picks
+    from oracle import numpy_oracle
+    from tensororder_amd import plan_io
+
+    f = parse_dimacs_text(random_3cnf(22, 44, 5, True), "cachet")
+    net = network.cnf_count(f)
+    tds = planners.decompose(net.structure_text(), None, tie_seeds=[0, 1])
+    plans, labels = planners.plan_candidates(net, "factor", tds, 3)
+    assert len(plans) >= 4 and all(len(p.groups_to_slice) == 3 for p in plans)
+    ref = planners.reference_candidate(labels)
+    assert labels[ref]["slicer"] == "greedy_mem"
+    assert labels[ref]["unsliced_maxrank"] == min(l["unsliced_maxrank"] for l in labels)
+    counts = [numpy_oracle.contract_sliced(plan_io.plan_to_dict(p)) for p in plans]
+    np.testing.assert_allclose(counts, counts[ref], rtol=1e-12)
+
+
+def test_candidate_fixture_rebuilds():
+    from tensororder_amd import train_costmodel
+
+    plans, labels, ref, doc = train_costmodel.load_candidates(CANDIDATES)
+    assert labels == doc["candidates"] and ref == doc["reference"]
+    assert all(len(p.groups_to_slice) == doc["slices"] for p in plans)
+
+
+@pytest.mark.gpu
+def test_trained_model_choice_beats_reference_rule():
+    # the committed model picks among the 24 candidate plans of the 4096-slice formula;
+    # both picks run in full with the FP32-scaled executor: same count, less time
+    from tensororder_amd import train_costmodel
+    from tensororder_amd.hip_api import HipF32API
+
+    api = HipF32API()
+    model = costmodel.load_default(api)
+    plans, labels, ref, doc = train_costmodel.load_candidates(CANDIDATES)
+    choice, preds = costmodel.learned_choice(model, api, plans)
+    print("reference #%d predicted %.3f s; learned #%d predicted %.3f s" % (ref, preds[ref], choice, preds[choice]),
+          flush=True)
+    assert preds[choice] <= preds[ref]
+    t_ref, v_ref = train_costmodel.measure(api, plans[ref], warmup=0)
+    t_new, v_new = train_costmodel.measure(api, plans[choice], warmup=0)
+    print("measured: reference %.3f s, learned %.3f s" % (t_ref, t_new), flush=True)
+    assert abs(v_new - v_ref) <= 1e-6 * abs(v_ref)
+    assert t_new < t_ref
