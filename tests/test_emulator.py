@@ -18,7 +18,7 @@ from tensororder_amd.tree import TreeContext
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SMALL = [f for f in sorted(glob.glob(os.path.join(HERE, "golden", "*.json")))
-         if json.load(open(f))["meta"]["maxrank"] <= 16]
+         if not f.endswith("_candidates.json") and json.load(open(f))["meta"]["maxrank"] <= 16]
 
 
 @pytest.mark.parametrize("path", SMALL, ids=[os.path.basename(p)[:-5] for p in SMALL])
