@@ -62,18 +62,33 @@ def kernel_symbol(L, scaled):
             _lib.TO_OP_STEP_BEGIN: "k_step_begin", _lib.TO_OP_BATCH_BEGIN: "k_batch_begin"}.get(op, "op%d" % op)
 
 
-def roofline_of(name, k):
+def measured_traffic(workload, name):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size)
+    of one kernel on one workload, from the committed rocprofv3 counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get(workload, {}).get(name)
+    if rec is None:
+        return None
+    return rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"]
+
+
+def roofline_of(name, k, workload=None):
     """Roofline object of one kernel symbol from its summed launches of one step."""
     if k["flops"] and k["flops"] / max(k["bytes"], 1) > 20 and name.startswith("k_gemm"):
         peak = FP32_PEAK_TFLOPS if "<float" in name else FP64_PEAK_TFLOPS
         achieved = k["flops"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
         return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "kernel": name, "launches_per_step": k["launches"],
+                "frac": achieved / peak, "traffic": measured_traffic(workload, name), "kernel": name,
+                "launches_per_step": k["launches"],
                 "kernel_ms_per_step": k["ms"], "avg_launch_us": 1e3 * k["ms"] / max(k["launches"], 1),
                 "algorithmic_flops_per_step": k["flops"]}
     achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name, "launches_per_step": k["launches"],
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload, name), "kernel": name,
+            "launches_per_step": k["launches"],
             "kernel_ms_per_step": k["ms"], "avg_launch_us": 1e3 * k["ms"] / max(k["launches"], 1),
             "algorithmic_bytes_per_step": k["bytes"]}
 
@@ -269,6 +284,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-full", action="store_true", help="time the full unsliced oracle contraction")
+    ap.add_argument("--fp32-only", action="store_true",
+                    help="time only the FP32-scaled executor on the workload's plan (for profiler runs)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned plan choice where the workload has candidate plans")
     ap.add_argument("--permute-sweep", action="store_true",
@@ -299,7 +316,8 @@ def main():
     log("workload", args.workload, "maxrank", doc["meta"]["maxrank"], "groups", len(doc["groups_to_slice"]))
 
     results = {}
-    for name, cls in (("fp32s", HipF32API), ("fp64", HipAPI)):
+    executors = (("fp32s", HipF32API),) if args.fp32_only else (("fp32s", HipF32API), ("fp64", HipAPI))
+    for name, cls in executors:
         api = cls()
         plan = plan_io.plan_from_dict(doc)
         exe = api.prepare(plan)
@@ -313,9 +331,18 @@ def main():
         dt = time.perf_counter() - t0
         log(name, "steps", args.steps, "s", round(dt, 4), "count", res.value, "log10", res.log10)
         classes, symbols = per_kernel_times(api, exe)
-        results[name] = dict(api=api, exe=exe, dt=dt, res=res, classes=classes, symbols=symbols)
+        # release this executor's device memory before the next one sizes its waves
+        exe.close()
+        del exe
+        torch.cuda.empty_cache()
+        results[name] = dict(api=api, dt=dt, res=res, classes=classes, symbols=symbols)
         log(name, "kernels", {k: (round(v["ms"], 3), v["launches"]) for k, v in classes.items()})
 
+    if args.fp32_only:
+        r32 = results["fp32s"]
+        print(json.dumps({"workload": args.workload, "executor": "fp32-scaled", "steps": args.steps,
+                          "ms_per_step": r32["dt"] / args.steps * 1e3, "kernel_symbols": r32["symbols"]}), flush=True)
+        return
     r32, r64 = results["fp32s"], results["fp64"]
     learned = None
     cand_path = os.path.join(ROOT, "tests", "golden", args.workload.rsplit("_", 2)[0] + "_candidates.json")
@@ -328,10 +355,13 @@ def main():
     classes, symbols = r32["classes"], r32["symbols"]
     dom_name = max(symbols, key=lambda k: symbols[k]["ms"])
     dom = symbols[dom_name]
-    roofline = roofline_of(dom_name, dom)
+    roofline = roofline_of(dom_name, dom, args.workload)
     # the permute kernel's own HBM roofline (it is HBM-bound whatever dominates)
     perm = {k: v for k, v in symbols.items() if k.startswith("k_permute")}
-    perm_roofline = roofline_of(next(iter(perm)), perm[next(iter(perm))]) if perm else None
+    perm_roofline = None
+    if perm:
+        pname = max(perm, key=lambda k: perm[k]["ms"])
+        perm_roofline = roofline_of(pname, perm[pname], args.workload)
 
     value = args.steps / r32["dt"] if learned is None else learned["value"]
     out = {
@@ -366,8 +396,6 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(doc, args.cpu_budget, args.cpu_full)
-    for r in results.values():
-        r["exe"].close()
     print(json.dumps(out), flush=True)
 
 

@@ -131,7 +131,6 @@ class Executable:
         acc = _lib.Accum.from_buffer_copy(acc_bytes)
         res = Result(acc.mant, acc.exp, acc.count)
         api.last_result = res
-        api.last_program = self.prog
         api.last_stats = dict(self.prog.stats, nbatch=self.prog.nbatch, waves=self.nwaves, slices=self.total)
         return res
 
@@ -213,7 +212,6 @@ class HipAPI(BaseTensorAPI):
         self._use_graph = True
         self._regroup = True
         self.last_result = None
-        self.last_program = None
         self.last_stats = {}
 
     # -- plugin interface ---------------------------------------------------------
