@@ -30,6 +30,8 @@ WORKLOADS = {
     "cnf70_factor_s12": "random 3-CNF 70 vars / 140 clauses, w~U(0.1,0.9), factor-Flow plan sliced on "
                         "12 groups (4096 slices, max rank 25)",
     "vc90_weighted_line": "weighted vertex cover, random 3-regular graph n=90, line-Flow plan",
+    "vc_batch512": "mixed batch of 512 weighted vertex-cover instances, random 3-regular graphs with even "
+                   "n~U[30,150], line-Flow plans of max rank 6-22, one HIP graph per batch",
 }
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
@@ -275,6 +277,79 @@ def learned_choice_run(api, cand_path, args, fp64_value):
             "rel_err_vs_fp64_reference_plan": abs(res.value - fp64_value) / abs(fp64_value)}
 
 
+def batch_bench(args):
+    """The mixed batch: every instance of the batch fixture contracted by one grouped
+    schedule (one HIP graph launch per step); value = instances/s of the FP32-scaled
+    executor, with per-instance completion times (device clock, from the batch start)."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from tensororder_amd import _lib, plan_io
+    from tensororder_amd.hip_api import HipAPI, HipF32API
+
+    with open(os.path.join(ROOT, "tests", "golden", args.workload + ".json")) as f:
+        doc = json.load(f)
+    t0 = time.perf_counter()
+    plans, expected = plan_io.batch_plans(doc)
+    log("batch", len(plans), "plans rebuilt in %.1f s" % (time.perf_counter() - t0))
+    out = {}
+    for name, cls in (("fp32s", HipF32API), ("fp64", HipAPI)):
+        api = cls()
+        exe = api.prepare_batch(plans)
+        for _ in range(args.warmup):
+            exe.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res, lat = exe.run()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        rel = max(abs(r.value - e) / abs(e) for r, e in zip(res, expected))
+        lat_ms = np.asarray(lat) * 1e3
+        prog = exe.prog
+        arr = _lib.launch_array(prog.launches)
+        ms = (ctypes.c_float * len(prog.launches))()
+        _lib.check(api._lib.to_run_timed(ctypes.addressof(arr), len(prog.launches), api._entry_type,
+                                         ctypes.c_void_p(api._stream.cuda_stream), ms), "to_run_timed")
+        scaled = api._entry_type == _lib.TO_F32S
+        symbols = {}
+        for L, (b, fl), t in zip(prog.launches, prog.work, ms):
+            c = symbols.setdefault(kernel_symbol(L, scaled), {"ms": 0.0, "launches": 0, "bytes": 0, "flops": 0})
+            c["ms"] += t
+            c["launches"] += 1
+            c["bytes"] += b
+            c["flops"] += fl
+        out[name] = {"value": len(plans) * args.steps / dt, "ms_per_step": dt / args.steps * 1e3,
+                     "latency_ms_p50": float(np.percentile(lat_ms, 50)),
+                     "latency_ms_p99": float(np.percentile(lat_ms, 99)),
+                     "rel_err_vs_oracle_max": rel, "launches_per_step": len(prog.launches),
+                     "steps_per_batch": prog.stats["steps"], "device_ms_per_step": float(sum(ms)),
+                     "kernel_symbols": symbols}
+        log(name, "batch step %.3f ms, %.1f instances/s, p50 %.3f ms p99 %.3f ms, max rel err %.2e"
+            % (out[name]["ms_per_step"], out[name]["value"], out[name]["latency_ms_p50"],
+               out[name]["latency_ms_p99"], rel))
+        exe.close()
+        del exe
+        torch.cuda.empty_cache()
+    r32 = out["fp32s"]
+    dom_name = max(r32["kernel_symbols"], key=lambda k: r32["kernel_symbols"][k]["ms"])
+    line = {
+        "metric": METRIC, "value": r32["value"], "unit": "instances/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r32["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "description": WORKLOADS.get(args.workload, ""),
+                   "instances": len(plans), "executor": "fp32-scaled, grouped batch schedule in one HIP graph"},
+        "fp32s": {k: v for k, v in r32.items() if k != "kernel_symbols"},
+        "fp64": {k: v for k, v in out["fp64"].items() if k != "kernel_symbols"},
+        "fp32s_speedup_vs_fp64": r32["value"] / out["fp64"]["value"],
+        "roofline": roofline_of(dom_name, r32["kernel_symbols"][dom_name], args.workload),
+        "kernel_symbols": r32["kernel_symbols"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,6 +379,10 @@ def main():
               "(slices would shard across GPUs with one FP64 partial sum each); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload.startswith("vc_batch"):
+        batch_bench(args)
+        return
 
     import torch
 

@@ -47,7 +47,11 @@ extern "C" {
 
 /* launch record op codes */
 #define TO_OP_LEAF_GATHER 1  /* descs: to_leaf_desc[count] */
-#define TO_OP_JOIN_STREAM 2  /* descs: to_join_desc[count]; variant = y values per thread */
+#define TO_OP_JOIN_STREAM 2  /* descs: to_join_desc[count]; variant = y values per thread (YC),
+                                or TO_STREAM_VEC | YC for the 16-byte form: fx >= 10 (float) /
+                                9 (double), the lowest 2 / 1 bits of X free, shy == 0,
+                                shf == fy, YC == 2^fy, LDS 8*2^s (16-aligned) + YC*2^s entries */
+#define TO_STREAM_VEC 0x100
 #define TO_OP_JOIN_SPLITK 3  /* descs: to_join_desc[count]; partial sums to d.partial */
 #define TO_OP_JOIN_FINISH 4  /* descs: to_join_desc[count]; sums d.partial into C */
 #define TO_OP_SCALE_CLEAR 5  /* aux0: to_scale_rec*, iaux0: record count */

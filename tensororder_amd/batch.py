@@ -120,7 +120,13 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
             for kind, op, variant, lds, st, nblk in schedule.join_records(
                     n, nodes, 1, esize, scaled, abase, scratch.get(n.idx, {}), recp):
                 key = (KIND_ORDER[kind], op, variant)
-                steps.setdefault(k, {}).setdefault(key, []).append((st, nblk, lds, kind, i))
+                if kind == "permute":
+                    work = (2 * esize * (int(st.nbatch) << int(st.rank)), 0)
+                elif kind in ("stream", "splitk", "gemm"):
+                    work = schedule.join_traffic(n, nodes, 1, esize)
+                else:
+                    work = (0, 0)
+                steps.setdefault(k, {}).setdefault(key, []).append((st, nblk, lds, kind, i, work))
         root = nodes[-1]
         rd = _lib.RootDesc(root=root.ptr, rec=recp(root), acc=prog.accum.data_ptr() + i * acc_size)
         roots.setdefault(len(joins), []).append(rd)
@@ -138,39 +144,45 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
     pending = []
     pending.append((Launch(op=_lib.TO_OP_BATCH_BEGIN, grid=max(1, min(64, -(-len(per) // 256))),
                            aux2=prog.accum.data_ptr(), aux3=prog.start.data_ptr(), iaux0=len(per)), None,
-                    ("begin", -1)))
+                    ("begin", -1), (0, 0)))
     if scaled and nrec > join_rec0:
         pending.append((Launch(op=_lib.TO_OP_SCALE_CLEAR, grid=max(1, min(1024, -(-(nrec - join_rec0) // 256))),
-                               aux0=rbase + join_rec0 * rec_size, iaux0=nrec - join_rec0), None, ("clear", -1)))
+                               aux0=rbase + join_rec0 * rec_size, iaux0=nrec - join_rec0), None, ("clear", -1),
+                        (0, 0)))
     nsteps = max(list(steps) + list(roots)) + 1 if (steps or roots) else 0
     launches_per_kind = {}
     for k in range(nsteps):
         for key in sorted(steps.get(k, {})):
             group = steps[k][key]
-            structs, block0, lds = [], 0, 0
-            for st, nblk, l, kind, i in group:
+            structs, block0, lds, wb, wf = [], 0, 0, 0, 0
+            for st, nblk, l, kind, i, work in group:
                 st.block0 = block0
                 block0 += nblk
                 lds = max(lds, l)
                 structs.append(st)
+                wb += work[0]
+                wf += work[1]
             off = put(structs)
             kind = group[0][3]
             pending.append((Launch(op=key[1], variant=key[2], count=len(structs), grid=block0, lds=lds), off,
-                            (kind, k)))
+                            (kind, k), (wb, wf)))
             launches_per_kind[kind] = launches_per_kind.get(kind, 0) + 1
         if k in roots:
             rs = roots[k]
             off = put(rs)
             pending.append((Launch(op=_lib.TO_OP_ROOT_STORE, count=len(rs), grid=max(1, -(-len(rs) // 256))),
-                            off, ("root", k)))
+                            off, ("root", k), (0, 0)))
     prog.blob_host = np.frombuffer(bytes(blob) or b"\0", dtype=np.uint8).copy()
     prog.blob = device_alloc(prog.blob_host.nbytes)
     bbase = prog.blob.data_ptr()
-    for L, off, info in pending:
+    # per launch: (kind, step) and the algorithmic (bytes, flops) of its joins
+    prog.work = []
+    for L, off, info, work in pending:
         if off is not None:
             L.descs = bbase + off
         prog.launches.append(L)
         prog.info.append(info)
+        prog.work.append(work)
     prog.per = per
     prog.stats = dict(instances=len(per), steps=nsteps, launches=len(prog.launches),
                       joins=sum(sum(1 for n in p["nodes"] if not n.is_leaf) for p in per),

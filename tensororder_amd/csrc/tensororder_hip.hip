@@ -226,6 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
         const bool stage_in = (d->stage & 2) != 0;
         T* otile = ylds + YC * K;
         T* xtile = otile + (stage_out ? (kBlock << fy) : 0);
+        float mrun = 0.0f;
         for (int64_t t = t0; t < t1; ++t) {
             const int64_t key = batch * nych + ych;
             const int ny = min(YC, NY - ych * YC);
@@ -305,9 +306,16 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                 }
             }
             if (SCALED) {
-                m = wave_max(m);
-                if ((tid & 63) == 0) note_max(d->cs, batch, m);
+                // the max of a batch item's outputs is kept per thread and published
+                // (one atomic per wave) when the block moves to the next batch item
+                mrun = fmaxf(mrun, m);
                 if (tid == 0 && ft == 0 && ych == 0) note_exp(d->cs, batch, ex + ey);
+                const bool last = (t + 1 == t1) || (ych + 1 == nych && ft + 1 == nft);
+                if (last) {
+                    const float w = wave_max(mrun);
+                    if ((tid & 63) == 0) note_max(d->cs, batch, w);
+                    mrun = 0.0f;
+                }
             }
             if (++ych == nych) {
                 ych = 0;
@@ -333,6 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
         int64_t lt = t0 >> lnych;
         for (int64_t t = t0; t < t1; ++t) {
             const int64_t L = lt * kBlock + tid;
+            float m = 0.0f;
             if (L < nL) {
                 const int64_t batch = L >> fx;
                 const int ny = min(YC, NY - ych * YC);
@@ -364,10 +373,17 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                     if (yc < ny) {
                         const int64_t y = (int64_t)ych * YC + yc;
                         Cb[((int64_t)f << shf) | (y << shy)] = acc[yc];
-                        if (SCALED) note_max(d->cs, batch, (float)absval(acc[yc]));
+                        if (SCALED) m = fmaxf(m, (float)absval(acc[yc]));
                     }
                 }
                 if (SCALED && f == 0 && ych == 0) note_exp(d->cs, batch, ex + ey);
+            }
+            if (SCALED) {
+                // lanes of one batch item are an aligned group of 2^fx lanes (or the whole
+                // wave): reduce inside the group, its first lane publishes the max
+                const int group = fx < 6 ? (1 << fx) : 64;
+                for (int off = 1; off < group; off <<= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+                if (L < nL && (tid & (group - 1)) == 0) note_max(d->cs, L >> fx, m);
             }
             if (++ych == nych) {
                 ych = 0;
@@ -565,6 +581,137 @@ __global__ __launch_bounds__(kBlock) void k_root_reduce(const T* __restrict__ ro
     }
 }
 
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
+// ---------------------------------------------------------------------------------
+// streaming join, 16-byte form: the common skinny join whose X free bits start at bit 0
+// of X, whose output puts all 2^fy y values of one f next to each other (shy == 0,
+// shf == fy) and whose Y slab fits one chunk.  Each thread owns VF consecutive f, so it
+// loads 16-byte X chunks and stores its VF * 2^fy outputs as 16-byte chunks; a block walks
+// tiles of 256 * VF consecutive f of one batch item.
+
+template <typename T, bool SCALED, int YC>
+__global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* __restrict__ descs, int ndesc) {
+    constexpr int VF = 16 / sizeof(T);
+    constexpr int LVF = sizeof(T) == 4 ? 2 : 1;
+    typedef typename Vec16<T>::type V;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int tid = threadIdx.x;
+    const int fx = d->fx, fy = d->fy, s = d->s;
+    const int K = 1 << s;
+    const int nbatch = d->nbatch;
+    const TO_GLOBAL T* __restrict__ X = gptr<const T>(d->x);
+    const TO_GLOBAL T* __restrict__ Y = gptr<const T>(d->y);
+    TO_GLOBAL T* __restrict__ C = gptr<T>(d->c);
+    const int64_t xb = d->xb, yb = d->yb, cb = d->cb;
+
+    uint32_t* kx = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* ky = kx + K;
+    T* ylds = reinterpret_cast<T*>(smem + ((8 * K + 15) & ~15));
+    for (int k = tid; k < K; k += kBlock) {
+        kx[k] = (uint32_t)pdep(k, d->xs_pos, s);
+        ky[k] = (uint32_t)pdep(k, d->ys_pos, s);
+    }
+    __syncthreads();
+
+    const int local = blockIdx.x - d->block0;
+    const int nblk = d->nblocks;
+    const int tb = 8 + LVF;  // f bits inside a tile
+    const int hb = fx - tb;
+    const int64_t nft = 1ll << hb;
+    const int64_t ntiles = (int64_t)nbatch << hb;
+    const int64_t t0 = ntiles * local / nblk, t1 = ntiles * (local + 1) / nblk;
+    const int64_t xlo = pdep((uint64_t)tid << LVF, d->xf_pos, tb);
+    const int64_t hmask = pdep(~0ull, d->xf_pos + tb, hb);
+    int64_t ft = t0 & (nft - 1);
+    int64_t batch = t0 >> hb;
+    int64_t xhi = pdep(ft, d->xf_pos + tb, hb);
+    int64_t staged = -1;
+    float sx = 1.0f, sy = 1.0f;
+    int64_t ex = 0, ey = 0;
+    float mrun = 0.0f;
+    for (int64_t t = t0; t < t1; ++t) {
+        if (batch != staged) {
+            if (SCALED) {
+                const OpScale a = op_scale(d->xs, d->xsb, batch);
+                const OpScale c = op_scale(d->ys, d->ysb, batch);
+                sx = a.f; ex = a.e; sy = c.f; ey = c.e;
+            }
+            __syncthreads();
+            const auto Yb = Y + batch * yb;
+            for (int i = tid; i < YC * K; i += kBlock) {
+                const int yc = i >> s, k = i & (K - 1);
+                ylds[i] = Yb[pdep(yc, d->yf_pos, fy) + ky[k]] * T(sy);
+            }
+            __syncthreads();
+            staged = batch;
+        }
+        const auto Xp = X + batch * xb + xhi + xlo;
+        T acc[VF][YC];
+#pragma unroll
+        for (int i = 0; i < VF; ++i)
+#pragma unroll
+            for (int y = 0; y < YC; ++y) acc[i][y] = T(0);
+        for (int k0 = 0; k0 < K; k0 += 4) {
+            V xv[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                if (k0 + kk < K) xv[kk] = *reinterpret_cast<const TO_GLOBAL V*>(Xp + kx[k0 + kk]) * T(sx);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                if (k0 + kk < K) {
+#pragma unroll
+                    for (int y = 0; y < YC; ++y) {
+                        const T yv = ylds[y * K + k0 + kk];
+#pragma unroll
+                        for (int i = 0; i < VF; ++i) acc[i][y] += xv[kk][i] * yv;
+                    }
+                }
+            }
+        }
+        // outputs (f, y) of this thread: one run of VF * YC entries at f0 * YC
+        auto Cp = C + batch * cb + ((((ft << tb) | ((int64_t)tid << LVF))) * YC);
+        float m = 0.0f;
+#pragma unroll
+        for (int j = 0; j < VF * YC; j += VF) {
+            V o;
+#pragma unroll
+            for (int c = 0; c < VF; ++c) {
+                const T v = acc[(j + c) / YC][(j + c) % YC];
+                o[c] = v;
+                if (SCALED) m = fmaxf(m, (float)absval(v));
+            }
+            *reinterpret_cast<TO_GLOBAL V*>(Cp + j) = o;
+        }
+        if (SCALED) {
+            mrun = fmaxf(mrun, m);
+            if (tid == 0 && ft == 0) note_exp(d->cs, batch, ex + ey);
+            if (t + 1 == t1 || ft + 1 == nft) {
+                const float w = wave_max(mrun);
+                if ((tid & 63) == 0) note_max(d->cs, batch, w);
+                mrun = 0.0f;
+            }
+        }
+        if (++ft == nft) {
+            ft = 0;
+            xhi = 0;
+            ++batch;
+        } else {
+            xhi = ((xhi | ~hmask) + 1) & hmask;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // bit-permutation of all-dimension-2 tensors, LDS-tiled: a tile spans the union U of
 // the m lowest input bits and the input bits that land on the m lowest output bits
@@ -730,17 +877,6 @@ __device__ __forceinline__ void permute_tiles(const to_perm_desc* __restrict__ d
 
 template <typename T>
 constexpr int kPermVecBits = sizeof(T) == 4 ? 13 : 12;
-
-template <typename T>
-struct Vec16;
-template <>
-struct Vec16<float> {
-    typedef float type __attribute__((ext_vector_type(4)));
-};
-template <>
-struct Vec16<double> {
-    typedef double type __attribute__((ext_vector_type(2)));
-};
 
 template <typename T>
 __device__ __forceinline__ void permute_tiles_vec(const to_perm_desc* __restrict__ d, int local, int nblocks,
@@ -1170,9 +1306,12 @@ int launch_one(const to_launch& L, hipStream_t st) {
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
             const to_join_desc* d = reinterpret_cast<const to_join_desc*>(L.descs);
             switch (L.variant) {
-#define TO_STREAM_CASE(YC)                                                                     \
-    case YC:                                                                                   \
-        hipLaunchKernelGGL((k_join_stream<T, SCALED, YC>), grid, block, L.lds, st, d, L.count); \
+#define TO_STREAM_CASE(YC)                                                                         \
+    case YC:                                                                                       \
+        hipLaunchKernelGGL((k_join_stream<T, SCALED, YC>), grid, block, L.lds, st, d, L.count);     \
+        break;                                                                                     \
+    case TO_STREAM_VEC | YC:                                                                       \
+        hipLaunchKernelGGL((k_join_stream_vec<T, SCALED, YC>), grid, block, L.lds, st, d, L.count); \
         break;
                 TO_STREAM_CASE(1)
                 TO_STREAM_CASE(2)

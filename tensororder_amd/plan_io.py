@@ -119,3 +119,32 @@ def save_plan(plan, path, meta=None):
 def load_plan(path):
     with open(path) as f:
         return plan_from_dict(json.load(f))
+
+
+BATCH_FORMAT = "tensororder-batch/1"
+
+
+def batch_plans(doc):
+    """Plans of a ``tensororder-batch/1`` document: weighted vertex-cover instances, each
+    rebuilt from (n, graph seed) and its stored line-graph decomposition.  Returns
+    (plans, expected float64 counts)."""
+    import base64
+    import zlib
+
+    from . import decomposition, network, planners
+    from .formula import parse_dimacs_text
+    from .instances import vertex_cover_cnf
+    from .plan import SlicedExecutionPlan
+
+    if doc.get("format") != BATCH_FORMAT:
+        raise ValueError("not a %s document" % BATCH_FORMAT)
+    plans, expected = [], []
+    for row in doc["instances"]:
+        net = network.cnf_count(parse_dimacs_text(vertex_cover_cnf(row["n"], row["seed"], True), doc["weights"]))
+        text = zlib.decompress(base64.b64decode(row["decomposition"])).decode()
+        td = decomposition.parse_all(text)[0]
+        tree, nn = planners.plan_line(net, td)
+        tree.treewidth = td.width()
+        plans.append(SlicedExecutionPlan(tree, nn))
+        expected.append(row["expected"])
+    return plans, expected

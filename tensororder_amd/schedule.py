@@ -19,6 +19,7 @@ The reference walks the tree once per slice, calling tensordot on every join
 
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -208,12 +209,12 @@ def join_geometry(lrank, rrank, lmap, rmap):
                 xf=b_free, xs=b_sh, yf=a_free, ys=a_sh)
 
 
-import os
-
 # largest GEMM output tile edge to use (64 or 128)
 GEMM_TILES = int(os.environ.get("TENSORORDER_GEMM_TILES", "128"))
 # stage streamed-join outputs through LDS when they form one contiguous run per tile
 STAGE_OUTPUTS = os.environ.get("TENSORORDER_STAGE_OUTPUTS", "1") != "0"
+# the 16-byte streaming join form (TENSORORDER_STREAM_VEC=0 turns it off)
+STREAM_VEC = os.environ.get("TENSORORDER_STREAM_VEC", "1") != "0"
 # float GEMMs with at least this many shared bits accumulate partial sums in double
 WIDE_ACCUM_MIN_SHARED = 13
 GEMM_MIN_FREE = 6
@@ -241,6 +242,16 @@ def choose_variant(g, nb, esize):
                     perm_x=not _canonical(g["xf"], g["xs"]), perm_y=not _canonical(g["yf"], g["ys"]))
     yc = min(16, 1 << fy)
     stage = 0
+    # 16-byte form: each thread owns 16 bytes of consecutive f (X's lowest bits are free)
+    # and stores its 2^fy outputs per f as 16-byte chunks (all y of one f adjacent)
+    lvf = 2 if esize == 4 else 1
+    if STREAM_VEC and fx >= 8 + lvf and list(g["xf"][:lvf]) == list(range(lvf)) and g["shy"] == 0 \
+            and g["shf"] == fy and fy <= 4 and s <= STREAM_MAX_S and not (s >= 8 and (nb << fx) < 32768):
+        lds = ((8 * K + 15) & ~15) + (1 << fy) * K * esize
+        if lds <= LDS_LIMIT:
+            ntiles = nb << (fx - 8 - lvf)
+            return dict(kind="stream", yc=1 << fy, nych=1, lds=lds, stage=0, vec=True,
+                        nblocks=int(min(ntiles, MAX_STREAM_BLOCKS)))
     if fx >= 8:
         while yc > 1 and 8 * K + yc * K * esize > LDS_LIMIT:
             yc //= 2
@@ -267,7 +278,7 @@ def choose_variant(g, nb, esize):
             ntiles = nb * (1 << (fx - 8)) * nych
         else:
             ntiles = -(-(nb << fx) // 256) * nych
-        return dict(kind="stream", yc=yc, nych=nych, lds=lds, stage=stage,
+        return dict(kind="stream", yc=yc, nych=nych, lds=lds, stage=stage, vec=False,
                     nblocks=int(min(ntiles, MAX_STREAM_BLOCKS)))
     nout = nb << (fx + fy)
     nsplit = 1
@@ -507,7 +518,8 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
         d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
         d.lnych = v["nych"].bit_length() - 1
         d.stage = int(v["stage"])
-        out.append(("stream", _lib.TO_OP_JOIN_STREAM, v["yc"], v["lds"], d, v["nblocks"]))
+        variant = v["yc"] | (_lib.TO_STREAM_VEC if v["vec"] else 0)
+        out.append(("stream", _lib.TO_OP_JOIN_STREAM, variant, v["lds"], d, v["nblocks"]))
         return out
     d.partial = abase + scratch["partial"][0]
     d.nsplit = v["nsplit"]

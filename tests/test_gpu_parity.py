@@ -215,3 +215,19 @@ def test_batch_of_uneven_instances(hip, hip32):
                 assert all(l >= 0 for l in lat)
         finally:
             exe.close()
+
+
+def test_batch_fixture_instances(hip, hip32):
+    # the first 64 instances of the 512-instance mixed batch (FlowCutter line plans of
+    # max rank 6-22) against the oracle counts stored with the fixture
+    doc = json.load(open(os.path.join(HERE, "golden", "vc_batch512.json")))
+    doc = dict(doc, instances=doc["instances"][:64])
+    plans, wants = plan_io.batch_plans(doc)
+    _say("case batch fixture (64 instances)")
+    for api, tol in ((hip, F64_TOL), (hip32, F32_TOL)):
+        exe = api.prepare_batch(plans)
+        try:
+            res, lat = exe.run()
+            assert [r.value for r in res] == pytest.approx(wants, rel=tol)
+        finally:
+            exe.close()
