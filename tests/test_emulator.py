@@ -2,12 +2,12 @@
 reproduce the oracle: bit placement, batching, slice-row gather, memory plan, split-K and
 the (mantissa, exponent) slice sum, in both entry types."""
 
-import glob
 import json
 import os
 
 import numpy as np
 import pytest
+from fixtures import plan_fixture_paths
 
 import emulator
 from oracle import numpy_oracle
@@ -17,8 +17,7 @@ from tensororder_amd.plan import SlicedExecutionPlan
 from tensororder_amd.tree import TreeContext
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SMALL = [f for f in sorted(glob.glob(os.path.join(HERE, "golden", "*.json")))
-         if not f.endswith("_candidates.json") and json.load(open(f))["meta"]["maxrank"] <= 16]
+SMALL = [f for f in plan_fixture_paths() if json.load(open(f))["meta"]["maxrank"] <= 16]
 
 
 @pytest.mark.parametrize("path", SMALL, ids=[os.path.basename(p)[:-5] for p in SMALL])

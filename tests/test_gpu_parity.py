@@ -4,7 +4,6 @@ Tolerances: plain FP64 executor rel <= 1e-12 of the FP64 oracle (exact for unwei
 counts < 2^53); FP32-scaled executor rel <= 1e-6.
 """
 
-import glob
 import itertools
 import json
 import os
@@ -12,6 +11,7 @@ import sys
 
 import numpy as np
 import pytest
+from fixtures import plan_fixture_paths
 
 from oracle import numpy_oracle
 from tensororder_amd import plan_io
@@ -21,8 +21,7 @@ from tensororder_amd.plan import ALL_SLICERS
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# plan fixtures (the candidate-set fixture holds decompositions, not a plan)
-FIXTURES = [f for f in sorted(glob.glob(os.path.join(HERE, "golden", "*.json"))) if not f.endswith("_candidates.json")]
+FIXTURES = plan_fixture_paths()
 SMALL = [f for f in FIXTURES if json.load(open(f))["meta"]["maxrank"] <= 20]
 F64_TOL = 1e-12
 F32_TOL = 1e-6

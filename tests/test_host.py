@@ -201,10 +201,10 @@ def test_plan_roundtrip_and_fixtures():
     back = plan_io.plan_from_dict(json.loads(json.dumps(d)))
     assert plan_io.plan_to_dict(back) == d
     assert back.maxrank == plan.maxrank and back.memory == plan.memory
-    for path in sorted(os.listdir(os.path.join(HERE, "golden"))):
-        if not path.endswith(".json") or path.endswith("_candidates.json"):
-            continue
-        doc = json.load(open(os.path.join(HERE, "golden", path)))
+    from fixtures import plan_fixture_paths
+
+    for path in plan_fixture_paths():
+        doc = json.load(open(path))
         p = plan_io.plan_from_dict(doc)
         assert p.maxrank == doc["meta"]["maxrank"]
         assert is_tree_complete(p.tree, p.network)
