@@ -306,7 +306,9 @@ def batch_bench(args):
             res, lat = exe.run()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        rel = max(abs(r.value - e) / abs(e) for r, e in zip(res, expected))
+        errs = [abs(r.value - e) / abs(e) for r, e in zip(res, expected)]
+        rel = max(errs)
+        worst = sorted(range(len(errs)), key=lambda i: -errs[i])[:5]
         lat_ms = np.asarray(lat) * 1e3
         prog = exe.prog
         arr = _lib.launch_array(prog.launches)
@@ -324,7 +326,12 @@ def batch_bench(args):
         out[name] = {"value": len(plans) * args.steps / dt, "ms_per_step": dt / args.steps * 1e3,
                      "latency_ms_p50": float(np.percentile(lat_ms, 50)),
                      "latency_ms_p99": float(np.percentile(lat_ms, 99)),
-                     "rel_err_vs_oracle_max": rel, "launches_per_step": len(prog.launches),
+                     "rel_err_vs_oracle_max": rel,
+                     "rel_err_vs_oracle_median": float(np.median(errs)),
+                     "worst_instances": [{"index": i, "n": doc["instances"][i]["n"],
+                                          "maxrank": doc["instances"][i]["maxrank"], "rel_err": errs[i]}
+                                         for i in worst],
+                     "launches_per_step": len(prog.launches),
                      "steps_per_batch": prog.stats["steps"], "device_ms_per_step": float(sum(ms)),
                      "kernel_symbols": symbols}
         log(name, "batch step %.3f ms, %.1f instances/s, p50 %.3f ms p99 %.3f ms, max rel err %.2e"

@@ -52,7 +52,7 @@ extern "C" {
                                 9 (double), the lowest 2 / 1 bits of X free, shy == 0,
                                 shf == fy, YC == 2^fy, LDS 8*2^s (16-aligned) + YC*2^s entries */
 #define TO_STREAM_VEC 0x100
-#define TO_OP_JOIN_SPLITK 3  /* descs: to_join_desc[count]; partial sums to d.partial */
+#define TO_OP_JOIN_SPLITK 3  /* descs: to_join_desc[count]; partial sums (double) to d.partial */
 #define TO_OP_JOIN_FINISH 4  /* descs: to_join_desc[count]; sums d.partial into C */
 #define TO_OP_SCALE_CLEAR 5  /* aux0: to_scale_rec*, iaux0: record count */
 #define TO_OP_ROOT_REDUCE 6  /* aux0: root values, aux1: to_scale_rec* (F32S) or 0,
@@ -76,7 +76,8 @@ typedef struct to_join_desc {
     int64_t xb, yb, cb;        /* batch strides in elements (0 = broadcast) */
     uint64_t xs, ys, cs;       /* to_scale_rec* per batch item (TO_F32S), else 0 */
     int64_t xsb, ysb;          /* scale-record batch strides (0 or 1) */
-    uint64_t partial;          /* split-K partial sums (TO_OP_JOIN_SPLITK / _FINISH) */
+    uint64_t partial;          /* split-K partial sums, double, nsplit per output
+                                  (TO_OP_JOIN_SPLITK / _FINISH) */
     int32_t fx, fy, s;         /* X free bits, Y free bits, shared bits */
     int32_t shf, shy;          /* output bit offset of f and of y */
     int32_t nbatch;            /* batch items */

@@ -59,10 +59,9 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
             if n.is_leaf and n.tensor not in inst["table_off"]:
                 rows = inst["tables"][n.tensor].rows
                 if scaled:
-                    m = float(np.max(np.abs(rows))) if rows.size else 0.0
-                    e = math.frexp(m)[1] if m > 0 else 0
+                    rows, F, e = schedule.leaf_encoding(rows)
                     inst["exp0"][n.tensor] = e
-                    rows = np.ldexp(rows, -e)
+                    inst.setdefault("factors", []).append(F)
                 arr = np.ascontiguousarray(rows[:1], dtype=fdtype).ravel()
                 inst["table_off"][n.tensor] = off
                 chunks.append((off, arr))
@@ -70,6 +69,8 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
     table_host = np.zeros(max(off, 1), dtype=fdtype)
     for o, arr in chunks:
         table_host[o:o + arr.size] = arr
+    # per instance: product of its leaves' encoding factors (mantissa, exponent)
+    prog.leaf_factors = [schedule.factor_product(inst.get("factors", [])) for inst in per]
     # scale records: constant leaves first (initialised once), then joins (cleared per run)
     nrec, rec_init = 0, []
     if scaled:

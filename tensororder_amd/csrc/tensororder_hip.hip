@@ -139,6 +139,12 @@ __device__ __forceinline__ T absval(T v) {
     return v < T(0) ? -v : v;
 }
 
+// The general streaming joins accumulate float operands in double: the products of two
+// floats are exact in double and the output is rounded once, when it is stored.  These
+// kernels are bound by HBM bytes, so the double arithmetic costs nothing measurable.
+template <typename T>
+using Acc = double;
+
 // ---------------------------------------------------------------------------------
 // leaf gather: per batch item, copy row sum_g coef[g]*bit_g(slice) of each leaf table
 
@@ -259,9 +265,9 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                 for (int i = tid; i < (kBlock << s); i += kBlock) xtile[i] = src[i];
                 __syncthreads();
             }
-            T acc[YC];
+            Acc<T> acc[YC];
 #pragma unroll
-            for (int yc = 0; yc < YC; ++yc) acc[yc] = T(0);
+            for (int yc = 0; yc < YC; ++yc) acc[yc] = 0.0;
             for (int k0 = 0; k0 < K; k0 += 8) {
                 T xv[8];
                 if (stage_in) {
@@ -275,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                 for (int yc = 0; yc < YC; ++yc) {
 #pragma unroll
                     for (int kk = 0; kk < 8; ++kk) {
-                        if (k0 + kk < K) acc[yc] += xv[kk] * ylds[yc * K + k0 + kk];
+                        if (k0 + kk < K) acc[yc] += (Acc<T>)xv[kk] * (Acc<T>)ylds[yc * K + k0 + kk];
                     }
                 }
             }
@@ -284,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
             if (SCALED) {
 #pragma unroll
                 for (int yc = 0; yc < YC; ++yc)
-                    if (yc < ny) m = fmaxf(m, (float)absval(acc[yc]));
+                    if (yc < ny) m = fmaxf(m, (float)absval((T)acc[yc]));
             }
             if (stage_out) {
                 // the tile's outputs are one contiguous run of 256 * 2^fy entries: stage
@@ -292,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                 __syncthreads();
 #pragma unroll
                 for (int yc = 0; yc < YC; ++yc)
-                    if (yc < NY) otile[tid * NY + yc] = acc[yc];
+                    if (yc < NY) otile[tid * NY + yc] = (T)acc[yc];
                 __syncthreads();
                 auto dst = Cb + (ft << (8 + fy));
                 for (int i = tid; i < (kBlock << fy); i += kBlock) dst[i] = otile[i];
@@ -301,7 +307,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                 for (int yc = 0; yc < YC; ++yc) {
                     if (yc < ny) {
                         const int64_t y = (int64_t)ych * YC + yc;
-                        Cb[(f << shf) | (y << shy)] = acc[yc];
+                        Cb[(f << shf) | (y << shy)] = (T)acc[yc];
                     }
                 }
             }
@@ -357,14 +363,14 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                 int64_t yoff[YC];
 #pragma unroll
                 for (int yc = 0; yc < YC; ++yc) yoff[yc] = (yc < ny) ? pdep(ych * YC + yc, d->yf_pos, fy) : 0;
-                T acc[YC];
+                Acc<T> acc[YC];
 #pragma unroll
-                for (int yc = 0; yc < YC; ++yc) acc[yc] = T(0);
+                for (int yc = 0; yc < YC; ++yc) acc[yc] = 0.0;
                 for (int k = 0; k < K; ++k) {
                     const T xv = Xp[kx[k]] * T(sx);
 #pragma unroll
                     for (int yc = 0; yc < YC; ++yc) {
-                        if (yc < ny) acc[yc] += xv * (Yb[yoff[yc] + ky[k]] * T(sy));
+                        if (yc < ny) acc[yc] += (Acc<T>)xv * (Acc<T>)(Yb[yoff[yc] + ky[k]] * T(sy));
                     }
                 }
                 auto Cb = C + batch * cb;
@@ -372,8 +378,9 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
                 for (int yc = 0; yc < YC; ++yc) {
                     if (yc < ny) {
                         const int64_t y = (int64_t)ych * YC + yc;
-                        Cb[((int64_t)f << shf) | (y << shy)] = acc[yc];
-                        if (SCALED) m = fmaxf(m, (float)absval(acc[yc]));
+                        const T v = (T)acc[yc];
+                        Cb[((int64_t)f << shf) | (y << shy)] = v;
+                        if (SCALED) m = fmaxf(m, (float)absval(v));
                     }
                 }
                 if (SCALED && f == 0 && ych == 0) note_exp(d->cs, batch, ex + ey);
@@ -398,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
 
 template <typename T, bool SCALED>
 __global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __restrict__ descs, int ndesc) {
-    __shared__ T red[kBlock / 64];
+    __shared__ double red[kBlock / 64];
     const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
     const int tid = threadIdx.x;
     const int fx = d->fx, fy = d->fy, s = d->s;
@@ -416,20 +423,20 @@ __global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __re
         sx = op_scale(d->xs, d->xsb, batch).f;
         sy = op_scale(d->ys, d->ysb, batch).f;
     }
-    T sum = T(0);
+    double sum = 0.0;
     if (s >= 8) {
         const int64_t chunk_hi = (1ll << (s - 8)) / nsplit;  // 256-wide steps per split
         const int64_t xlo = pdep(tid, d->xs_pos, 8), ylo = pdep(tid, d->ys_pos, 8);
         for (int64_t kh = sp * chunk_hi; kh < (sp + 1) * chunk_hi; ++kh) {
             const int64_t xo = pdep(kh, d->xs_pos + 8, s - 8) + xlo;
             const int64_t yo = pdep(kh, d->ys_pos + 8, s - 8) + ylo;
-            sum += (X[xo] * T(sx)) * (Y[yo] * T(sy));
+            sum += (double)(X[xo] * T(sx)) * (double)(Y[yo] * T(sy));
         }
     } else if (tid < (1 << s)) {
-        sum = (X[pdep(tid, d->xs_pos, s)] * T(sx)) * (Y[pdep(tid, d->ys_pos, s)] * T(sy));
+        sum = (double)(X[pdep(tid, d->xs_pos, s)] * T(sx)) * (double)(Y[pdep(tid, d->ys_pos, s)] * T(sy));
     }
     sum = block_sum(sum, red);
-    if (tid == 0) gptr<T>(d->partial)[o * nsplit + sp] = sum;
+    if (tid == 0) gptr<double>(d->partial)[o * nsplit + sp] = sum;
 }
 
 template <typename T, bool SCALED>
@@ -438,12 +445,13 @@ __global__ __launch_bounds__(kBlock) void k_join_finish(const to_join_desc* __re
     const int fx = d->fx, fy = d->fy;
     const int nsplit = d->nsplit;
     const int64_t nout = (int64_t)d->nbatch << (fx + fy);
-    const TO_GLOBAL T* partial = gptr<const T>(d->partial);
+    const TO_GLOBAL double* partial = gptr<const double>(d->partial);
     TO_GLOBAL T* C = gptr<T>(d->c);
     const int64_t stride = (int64_t)d->nblocks * kBlock;
     for (int64_t o = (int64_t)(blockIdx.x - d->block0) * kBlock + threadIdx.x; o < nout; o += stride) {
-        T sum = T(0);
-        for (int sp = 0; sp < nsplit; ++sp) sum += partial[o * nsplit + sp];
+        double acc = 0.0;
+        for (int sp = 0; sp < nsplit; ++sp) acc += partial[o * nsplit + sp];
+        const T sum = (T)acc;
         const int64_t batch = o >> (fx + fy);
         const int64_t f = (o >> fy) & ((1ll << fx) - 1);
         const int64_t y = o & ((1ll << fy) - 1);
@@ -657,6 +665,8 @@ __global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* 
             staged = batch;
         }
         const auto Xp = X + batch * xb + xhi + xlo;
+        // float accumulators here: the 16-byte form keeps VF * YC sums live, which in
+        // double would cost the occupancy this HBM-bound kernel needs
         T acc[VF][YC];
 #pragma unroll
         for (int i = 0; i < VF; ++i)
@@ -674,7 +684,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* 
                     for (int y = 0; y < YC; ++y) {
                         const T yv = ylds[y * K + k0 + kk];
 #pragma unroll
-                        for (int i = 0; i < VF; ++i) acc[i][y] += xv[kk][i] * yv;
+                        for (int i = 0; i < VF; ++i) acc[i][y] = fma(xv[kk][i], yv, acc[i][y]);
                     }
                 }
             }

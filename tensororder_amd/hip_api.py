@@ -129,7 +129,8 @@ class Executable:
         with api._torch.cuda.stream(api._stream):
             acc_bytes = self.prog.accum[: ctypes.sizeof(_lib.Accum)].cpu().numpy().tobytes()
         acc = _lib.Accum.from_buffer_copy(acc_bytes)
-        res = Result(acc.mant, acc.exp, acc.count)
+        fm, fe = getattr(self.prog, "leaf_factor", (1.0, 0))
+        res = Result(acc.mant * fm, acc.exp + fe, acc.count)
         api.last_result = res
         api.last_stats = dict(self.prog.stats, nbatch=self.prog.nbatch, waves=self.nwaves, slices=self.total)
         return res
@@ -181,7 +182,8 @@ class BatchExecutable:
             raw = self.prog.accum[: n * size].cpu().numpy().tobytes()
             start = int(self.prog.start[:8].cpu().numpy().view(np.int64)[0])
         accs = [_lib.Accum.from_buffer_copy(raw[i * size:(i + 1) * size]) for i in range(n)]
-        res = [Result(a.mant, a.exp, a.count) for a in accs]
+        res = [Result(a.mant * fm, a.exp + fe, a.count)
+               for a, (fm, fe) in zip(accs, getattr(self.prog, "leaf_factors", [(1.0, 0)] * n))]
         lat = [(a.pad - start) / self.clock_hz for a in accs]
         return res, lat
 
@@ -497,7 +499,7 @@ class HipAPI(BaseTensorAPI):
                                         grid=v["nblocks"], lds=v["lds"], descs=blob.data_ptr()))
             keep.append(blob)
         else:
-            part = self._alloc(v["nout"] * v["nsplit"] * self._esize)
+            part = self._alloc(v["nout"] * v["nsplit"] * 8)  # double partial sums
             d.partial = part.data_ptr()
             d.nsplit, d.nblocks = v["nsplit"], v["nblocks"]
             d2 = _lib.JoinDesc.from_buffer_copy(bytes(d))

@@ -213,10 +213,12 @@ def join_geometry(lrank, rrank, lmap, rmap):
 GEMM_TILES = int(os.environ.get("TENSORORDER_GEMM_TILES", "128"))
 # stage streamed-join outputs through LDS when they form one contiguous run per tile
 STAGE_OUTPUTS = os.environ.get("TENSORORDER_STAGE_OUTPUTS", "1") != "0"
+# the MFMA path for dense joins (TENSORORDER_GEMM=0 turns it off)
+USE_GEMM = os.environ.get("TENSORORDER_GEMM", "1") != "0"
 # the 16-byte streaming join form (TENSORORDER_STREAM_VEC=0 turns it off)
 STREAM_VEC = os.environ.get("TENSORORDER_STREAM_VEC", "1") != "0"
 # float GEMMs with at least this many shared bits accumulate partial sums in double
-WIDE_ACCUM_MIN_SHARED = 13
+WIDE_ACCUM_MIN_SHARED = int(os.environ.get("TENSORORDER_WIDE_ACCUM_MIN_SHARED", "11"))
 GEMM_MIN_FREE = 6
 GEMM_MIN_SHARED = 4
 MAX_GEMM_BLOCKS = 8192
@@ -232,7 +234,7 @@ def choose_variant(g, nb, esize):
     """Kernel variant and launch shape for a join with ``nb`` batch items."""
     fx, fy, s = g["fx"], g["fy"], g["s"]
     K = 1 << s
-    if fx >= GEMM_MIN_FREE and fy >= GEMM_MIN_FREE and s >= GEMM_MIN_SHARED:
+    if USE_GEMM and fx >= GEMM_MIN_FREE and fy >= GEMM_MIN_FREE and s >= GEMM_MIN_SHARED:
         big = GEMM_TILES == 128 and fx >= 7 and fy >= 7 and s >= (4 if esize == 8 else 5)
         tb = 7 if big else 6
         tiles = nb << (fx - tb + fy - tb)
@@ -373,6 +375,53 @@ def perm_desc(src, dst, rank, out_pos, nbatch, stride, esize):
     return d, d.nblocks, lds, 1 if vec else 0
 
 
+def leaf_encoding(rows):
+    """Float32 encoding of one leaf table for the scaled executor: (stored values as float64
+    that float32 holds exactly, FP64 factor F, power-of-two exponent e) with
+    rows ~= F * 2^e * stored.
+
+    A table whose nonzero magnitudes take at most two values -- a variable's weight pair
+    (w(not x), w(x)), or 0/1 clause entries -- is stored exactly: the two magnitudes become
+    the integers a, b < 2^24 of the best rational approximation b/a of their ratio, and F
+    carries the rest (relative error of the ratio ~1e-13).  Every leaf enters each slice's
+    contraction exactly once, so the whole count scales by the product of the leaves' F,
+    which the host divides out in FP64.  Other tables are rounded to float32 (F = 1)."""
+    rows = np.asarray(rows, dtype=np.float64)
+    mags = np.unique(np.abs(rows[rows != 0]))
+    F = 1.0
+    stored = rows
+    if len(mags) == 1:
+        F = float(mags[0])
+        stored = np.sign(rows)
+    elif len(mags) == 2:
+        import fractions
+
+        lo, hi = float(mags[0]), float(mags[1])
+        r = hi / lo
+        limit = int((1 << 24) - 1)
+        fr = fractions.Fraction(r).limit_denominator(max(1, int(limit // r)))
+        a, b = fr.denominator, fr.numerator
+        if 0 < a <= limit and 0 < b <= limit and abs(lo / a * b - hi) <= 1e-12 * hi:
+            F = lo / a
+            stored = np.where(np.abs(rows) == lo, np.sign(rows) * a, np.sign(rows) * b)
+            stored = np.where(rows == 0, 0.0, stored)
+    m = float(np.max(np.abs(stored))) if stored.size else 0.0
+    e = math.frexp(m)[1] if m > 0 else 0
+    return np.ldexp(stored, -e), F, e
+
+
+def factor_product(factors):
+    """Product of FP64 factors as (mantissa in [0.5, 1), int exponent), without overflow."""
+    mant, exp = 1.0, 0
+    for f in factors:
+        fm, fe = math.frexp(f)
+        mant *= fm
+        exp += fe
+        mant, e2 = math.frexp(mant)
+        exp += e2
+    return mant, exp
+
+
 def _pos_array(vals):
     arr = _lib.c_i8x32()
     for i, v in enumerate(vals):
@@ -386,7 +435,7 @@ def _temporaries(arena, n, nodes, W, esize, partial):
     v = n.geom["var"]
     blocks = []
     if v["kind"] == "splitk":
-        blocks.append(("partial", arena.alloc(v["nout"] * v["nsplit"] * esize)))
+        blocks.append(("partial", arena.alloc(v["nout"] * v["nsplit"] * 8)))  # double partial sums
     elif v["kind"] == "gemm":
         g = n.geom
         X, Y = (nodes[n.left], nodes[n.right]) if g["x_is_left"] else (nodes[n.right], nodes[n.left])
@@ -553,17 +602,18 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
     peak, partial = plan_memory(nodes, tables, W, esize)
     prog.arena_bytes = peak
 
-    # ---- leaf tables (host) -> one device buffer
+    # ---- leaf tables (host) -> one device buffer; the scaled executor's leaf factors
+    # multiply the count (prog.leaf_factor, mantissa and exponent)
     leaf_ids = sorted({n.tensor for n in nodes if n.is_leaf})
     table_off, chunks, exp0 = {}, [], {}
+    factors = []
     off = 0
     for t in leaf_ids:
         rows = tables[t].rows
         if scaled:
-            m = float(np.max(np.abs(rows))) if rows.size else 0.0
-            e = math.frexp(m)[1] if m > 0 else 0
+            rows, F, e = leaf_encoding(rows)
             exp0[t] = e
-            rows = np.ldexp(rows, -e)
+            factors.append(F)
         arr = np.ascontiguousarray(rows, dtype=fdtype).ravel()
         table_off[t] = off
         chunks.append(arr)
@@ -571,6 +621,7 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
     table_host = np.zeros(max(off, 1), dtype=fdtype)
     for t, arr in zip(leaf_ids, chunks):
         table_host[table_off[t]:table_off[t] + arr.size] = arr
+    prog.leaf_factor = factor_product(factors)
     coef_host = np.zeros(max(1, len(leaf_ids) * max(1, num_groups)), dtype=np.int64)
     coef_off = {}
     for i, t in enumerate(leaf_ids):

@@ -65,6 +65,17 @@ def _scale_exp(bits):
     return math.frexp(m)[1] if m > 0 else 0
 
 
+def _sum_mexp(terms):
+    """Sum of values m * 2^e given as (m, e) pairs, as a normalised (mantissa, exponent)."""
+    terms = [(m, e) for m, e in terms if m != 0.0]
+    if not terms:
+        return 0.0, 0
+    top = max(e for _, e in terms)
+    total = sum(math.ldexp(m, e - top) for m, e in terms)
+    m, e = math.frexp(total)
+    return m, (e + top if m else 0)
+
+
 class Emulator:
     def __init__(self, heap, entry_type):
         self.clock = 0
@@ -163,14 +174,14 @@ class Emulator:
         nvalid = max(0, min(int(L.iaux0), end - begin))
         vals = self.heap.view(L.aux0, self.dt, max(1, int(L.iaux0)))
         acc = self.heap.read_struct(_lib.Accum, L.aux2)
-        total = acc.mant * 2.0 ** acc.exp if acc.mant else 0.0
+        terms = [(acc.mant, acc.exp)]
         for b in range(nvalid):
-            v = float(vals[b])
+            m, e = math.frexp(float(vals[b]))
             if self.scaled:
-                v = v * 2.0 ** self.get_rec(L.aux1, b)[0]
-            total += v
-        m, e = math.frexp(total)
-        acc.mant, acc.exp, acc.count = m, (e if m else 0), acc.count + nvalid
+                e += self.get_rec(L.aux1, b)[0]
+            terms.append((m, e))
+        m, e = _sum_mexp(terms)
+        acc.mant, acc.exp, acc.count = m, e, acc.count + nvalid
         off = L.aux2 - HEAP_BASE
         self.heap.mem[off:off + ctypes.sizeof(_lib.Accum)] = np.frombuffer(bytes(acc), np.uint8)
 
@@ -221,10 +232,9 @@ class Emulator:
     def op_11(self, L):
         for i in range(L.count):
             d = self.heap.read_struct(_lib.RootDesc, L.descs + i * ctypes.sizeof(_lib.RootDesc))
-            v = float(self.heap.view(d.root, self.dt, 1)[0])
+            m, e = math.frexp(float(self.heap.view(d.root, self.dt, 1)[0]))
             if self.scaled:
-                v = v * 2.0 ** self.get_rec(d.rec, 0)[0]
-            m, e = math.frexp(v)
+                e += self.get_rec(d.rec, 0)[0]
             acc = _lib.Accum(mant=m, exp=e if m else 0, count=1, pad=self.clock)
             self.clock += 1
             off = d.acc - HEAP_BASE
@@ -259,7 +269,8 @@ def emulate_plan(tree, tables, num_groups, total, entry_type, W, regroup=True):
         for _ in range(-(-total // W)):
             emu.run(prog.wave)
     acc = heap.read_struct(_lib.Accum, prog.accum.data_ptr())
-    return math.ldexp(acc.mant, acc.exp), prog
+    fm, fe = getattr(prog, "leaf_factor", (1.0, 0))
+    return math.ldexp(acc.mant * fm, acc.exp + fe), prog
 
 
 def plan_tables(plan, api_like):
@@ -310,5 +321,6 @@ def emulate_batch(instances, entry_type):
     out = []
     for i in range(prog.count):
         acc = heap.read_struct(_lib.Accum, prog.accum.data_ptr() + i * ctypes.sizeof(_lib.Accum))
-        out.append(math.ldexp(acc.mant, acc.exp))
+        fm, fe = prog.leaf_factors[i] if hasattr(prog, "leaf_factors") else (1.0, 0)
+        out.append(math.ldexp(acc.mant * fm, acc.exp + fe))
     return out, prog
