@@ -53,7 +53,8 @@ extern "C" {
                                 shf == fy, YC == 2^fy, LDS 8*2^s (16-aligned) + YC*2^s entries */
 #define TO_STREAM_VEC 0x100
 #define TO_OP_JOIN_SPLITK 3  /* descs: to_join_desc[count]; partial sums (double) to d.partial */
-#define TO_OP_JOIN_FINISH 4  /* descs: to_join_desc[count]; sums d.partial into C */
+#define TO_OP_JOIN_FINISH 4  /* descs: to_join_desc[count]; sums d.partial into C; variant 1:
+                                one block per output (for nsplit >= 32) */
 #define TO_OP_SCALE_CLEAR 5  /* aux0: to_scale_rec*, iaux0: record count */
 #define TO_OP_ROOT_REDUCE 6  /* aux0: root values, aux1: to_scale_rec* (F32S) or 0,
                                 aux2: to_accum*, aux3: wave params (int64[2]) */

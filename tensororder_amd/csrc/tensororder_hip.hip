@@ -466,6 +466,38 @@ __global__ __launch_bounds__(kBlock) void k_join_finish(const to_join_desc* __re
     }
 }
 
+// Many partial sums per output (nsplit >= 32, e.g. a dot product of two large tensors):
+// one block per output, its threads sum strided partials and reduce in LDS.
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_join_finish_wide(const to_join_desc* __restrict__ descs, int ndesc) {
+    __shared__ double red[kBlock / 64];
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int fx = d->fx, fy = d->fy;
+    const int nsplit = d->nsplit;
+    const int64_t nout = (int64_t)d->nbatch << (fx + fy);
+    const TO_GLOBAL double* partial = gptr<const double>(d->partial);
+    TO_GLOBAL T* C = gptr<T>(d->c);
+    for (int64_t o = blockIdx.x - d->block0; o < nout; o += d->nblocks) {
+        double acc = 0.0;
+        for (int sp = threadIdx.x; sp < nsplit; sp += kBlock) acc += partial[o * nsplit + sp];
+        acc = block_sum(acc, red);
+        if (threadIdx.x == 0) {
+            const T sum = (T)acc;
+            const int64_t batch = o >> (fx + fy);
+            const int64_t f = (o >> fy) & ((1ll << fx) - 1);
+            const int64_t y = o & ((1ll << fy) - 1);
+            C[batch * d->cb + ((f << d->shf) | (y << d->shy))] = sum;
+            if (SCALED) {
+                note_max(d->cs, batch, (float)absval(sum));
+                if (f == 0 && y == 0) {
+                    const int64_t e = op_scale(d->xs, d->xsb, batch).e + op_scale(d->ys, d->ysb, batch).e;
+                    note_exp(d->cs, batch, e);
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // bookkeeping kernels
 
@@ -1182,14 +1214,15 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
 // BK (16 for double, 32 for float) with 16-byte global loads, and the next K slab is
 // fetched into registers while the current one is multiplied out of LDS.
 template <typename T, bool SCALED, bool WIDE>
-__global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restrict__ descs, int ndesc) {
-    constexpr int BM = 128, BN = 128, PAD = 4;
+__device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ descs, int ndesc) {
+    constexpr int BM = 128, BN = 128;
     constexpr int VE = 16 / sizeof(T);              // elements per 16-byte load
     constexpr int BK = sizeof(T) == 8 ? 16 : 32;
     constexpr int VPR = BK / VE;                     // 16-byte vectors per tile row
     constexpr int NV = BM * VPR / kBlock;            // vectors per thread per operand (4)
-    __shared__ T As[BK][BM + PAD];
-    __shared__ T Bs[BK][BN + PAD];
+    constexpr int LDK = BK + VE;                     // LDS row pitch: k contiguous, 16-byte pad
+    __shared__ __align__(16) T As[BM][LDK];
+    __shared__ __align__(16) T Bs[BN][LDK];
     typedef typename Mfma<T>::V V;
     typedef T L __attribute__((ext_vector_type(VE)));
     const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
@@ -1237,14 +1270,13 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
             rb[i] = *reinterpret_cast<const TO_GLOBAL L*>(B + vrow[i] * K + vk[i]);
         }
         for (int64_t k0 = 0; k0 < K; k0 += BK, ++step) {
+            // rows keep k contiguous, so each 16-byte slab goes to LDS in one store; the
+            // operands' power-of-two scales are applied to the outputs instead (exact)
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < NV; ++i) {
-#pragma unroll
-                for (int e = 0; e < VE; ++e) {
-                    As[vk[i] + e][vrow[i]] = ra[i][e] * sa;
-                    Bs[vk[i] + e][vrow[i]] = rb[i][e] * sb;
-                }
+                *reinterpret_cast<L*>(&As[vrow[i]][vk[i]]) = ra[i];
+                *reinterpret_cast<L*>(&Bs[vrow[i]][vk[i]]) = rb[i];
             }
             __syncthreads();
             if (k0 + BK < K) {
@@ -1259,9 +1291,9 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
                 const int kr = kk * 4 + (lane >> 4);
                 T a[4], b[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) a[i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
+                for (int i = 0; i < 4; ++i) a[i] = As[wm * 64 + i * 16 + (lane & 15)][kr];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) b[j] = Bs[kr][wn * 64 + j * 16 + (lane & 15)];
+                for (int j = 0; j < 4; ++j) b[j] = Bs[wn * 64 + j * 16 + (lane & 15)][kr];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -1272,6 +1304,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
 
         TO_GLOBAL T* C = gptr<T>(d->c) + batch * d->cb;
         float mx = 0.0f;
+        const T sab = sa * sb;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -1280,7 +1313,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int64_t m = tm * BM + wm * 64 + i * 16 + Mfma<T>::row(lane, v);
-                    const T out = kWideAccum<T, WIDE> ? T(wide[i][j][v]) : acc[i][j][v];
+                    const T out = (kWideAccum<T, WIDE> ? T(wide[i][j][v]) : acc[i][j][v]) * sab;
                     C[m * d->ldcm + n * d->ldcn] = out;
                     if (SCALED) mx = fmaxf(mx, (float)absval(out));
                 }
@@ -1292,6 +1325,12 @@ __global__ __launch_bounds__(kBlock) void k_gemm128(const to_gemm_desc* __restri
             if (tid == 0 && tm == 0 && tn == 0) note_exp(d->cs, batch, ea + eb);
         }
     }
+}
+
+// narrow accumulation fits two waves per SIMD; the wide form's double accumulators do not
+template <typename T, bool SCALED, bool WIDE>
+__global__ __launch_bounds__(kBlock, WIDE ? 1 : 2) void k_gemm128(const to_gemm_desc* __restrict__ descs, int ndesc) {
+    gemm128_body<T, SCALED, WIDE>(descs, ndesc);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1342,8 +1381,12 @@ int launch_one(const to_launch& L, hipStream_t st) {
             break;
         case TO_OP_JOIN_FINISH:
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
-            hipLaunchKernelGGL((k_join_finish<T, SCALED>), grid, block, 0, st,
-                               reinterpret_cast<const to_join_desc*>(L.descs), L.count);
+            if (L.variant == 1)
+                hipLaunchKernelGGL((k_join_finish_wide<T, SCALED>), grid, block, 0, st,
+                                   reinterpret_cast<const to_join_desc*>(L.descs), L.count);
+            else
+                hipLaunchKernelGGL((k_join_finish<T, SCALED>), grid, block, 0, st,
+                                   reinterpret_cast<const to_join_desc*>(L.descs), L.count);
             break;
         case TO_OP_SCALE_CLEAR:
             if (!L.aux0 || L.iaux0 <= 0) return TO_ERR_INVALID;

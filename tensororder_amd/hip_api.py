@@ -495,7 +495,8 @@ class HipAPI(BaseTensorAPI):
             d.stage = int(v["stage"])
             blob = self._alloc(ctypes.sizeof(d))
             self._upload(blob, bytes(d))
-            launches.append(_lib.Launch(op=_lib.TO_OP_JOIN_STREAM, variant=v["yc"], count=1,
+            variant = v["yc"] | (_lib.TO_STREAM_VEC if v["vec"] else 0)
+            launches.append(_lib.Launch(op=_lib.TO_OP_JOIN_STREAM, variant=variant, count=1,
                                         grid=v["nblocks"], lds=v["lds"], descs=blob.data_ptr()))
             keep.append(blob)
         else:
@@ -508,8 +509,8 @@ class HipAPI(BaseTensorAPI):
             self._upload(blob, bytes(d) + bytes(d2))
             launches.append(_lib.Launch(op=_lib.TO_OP_JOIN_SPLITK, count=1, grid=v["nblocks"],
                                         descs=blob.data_ptr()))
-            launches.append(_lib.Launch(op=_lib.TO_OP_JOIN_FINISH, count=1, grid=v["fin_blocks"],
-                                        descs=blob.data_ptr() + ctypes.sizeof(d)))
+            launches.append(_lib.Launch(op=_lib.TO_OP_JOIN_FINISH, variant=v["fin_variant"], count=1,
+                                        grid=v["fin_blocks"], descs=blob.data_ptr() + ctypes.sizeof(d)))
             keep += [blob, part]
         self._run(launches)
         self.synchronize()

@@ -286,8 +286,11 @@ def choose_variant(g, nb, esize):
     nsplit = 1
     if s >= 8:
         nsplit = 1 << int(min(s - 8, max(0, math.floor(math.log2(max(1, 4096 // nout))))))
+    # finishing: one thread per output, or one block per output when it has many partials
+    fin_wide = nsplit >= 32
+    fin_blocks = min(nout, MAX_STREAM_BLOCKS) if fin_wide else min(-(-nout // 256), MAX_STREAM_BLOCKS)
     return dict(kind="splitk", nsplit=nsplit, nout=nout, nblocks=int(nout * nsplit),
-                fin_blocks=int(min(-(-nout // 256), MAX_STREAM_BLOCKS)))
+                fin_blocks=int(fin_blocks), fin_variant=1 if fin_wide else 0)
 
 
 class LeafTable:
@@ -576,7 +579,7 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
     out.append(("splitk", _lib.TO_OP_JOIN_SPLITK, 0, 0, d, v["nblocks"]))
     d2 = JoinDesc.from_buffer_copy(bytes(d))
     d2.nblocks = v["fin_blocks"]
-    out.append(("finish", _lib.TO_OP_JOIN_FINISH, 0, 0, d2, v["fin_blocks"]))
+    out.append(("finish", _lib.TO_OP_JOIN_FINISH, v["fin_variant"], 0, d2, v["fin_blocks"]))
     return out
 
 
