@@ -63,8 +63,9 @@ extern "C" {
 #define TO_OP_PERMUTE 9      /* descs: to_perm_desc[count]; variant 1: 16-byte chunk kernel,
                                 which needs nu == 13 (float) / 12 (double) and 16-byte
                                 aligned buffers and strides; variant 0: nu <= 12 */
-#define TO_OP_GEMM 10        /* descs: to_gemm_desc[count]; variant bit 0: 128x128 output
-                                tiles (mlog, nlog >= 7) instead of 64x64 (>= 6); bit 1: float
+#define TO_OP_GEMM 10        /* descs: to_gemm_desc[count]; variant bit 0: 128-row output
+                                tiles (mlog >= 7), 128 columns (nlog >= 7) or, with bit 1,
+                                64 columns (nlog >= 6), instead of 64x64 (>= 6); bit 1: float
                                 partial sums accumulated in double every 4 K-steps */
 #define TO_OP_ROOT_STORE 11  /* descs: to_root_desc[count]: one instance's root -> its accumulator */
 #define TO_OP_BATCH_BEGIN 12 /* aux2: to_accum[iaux0] zeroed; aux3: int64 start time (device clock) */
@@ -141,8 +142,9 @@ typedef struct to_perm_desc {
 
 /* A batched dense product C[m][n] = sum_k A[m][k] * B[n][k]: A is 2^mlog x 2^klog and
  * B is 2^nlog x 2^klog, both row-major with k contiguous; C entry (m, n) is stored at
- * c + batch*cb + m*ldcm + n*ldcn.  Requires mlog, nlog >= 6 (>= 7 for 128x128 tiles)
- * and klog >= 4 (>= 5 for float with 128x128 tiles). */
+ * c + batch*cb + m*ldcm + n*ldcn.  Requires mlog, nlog >= 6 (mlog >= 7 and nlog >= 7, or
+ * nlog >= 6 for the wide form, with 128-row tiles) and klog >= 4 (>= 5 for float with
+ * 128-row tiles). */
 typedef struct to_gemm_desc {
     uint64_t a, b, c;
     int64_t ab, bb, cb;        /* batch strides in elements (0 = broadcast) */

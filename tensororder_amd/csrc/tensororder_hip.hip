@@ -1106,12 +1106,12 @@ struct Mfma<float> {
 template <typename T, bool WIDE>
 constexpr bool kWideAccum = WIDE && sizeof(T) == 4;
 
-template <typename T, int N>
-__device__ __forceinline__ void flush_wide(typename Mfma<T>::V (&acc)[N][N], double (&wide)[N][N][4]) {
+template <typename T, int N, int M>
+__device__ __forceinline__ void flush_wide(typename Mfma<T>::V (&acc)[N][M], double (&wide)[N][M][4]) {
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-        for (int j = 0; j < N; ++j)
+        for (int j = 0; j < M; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 wide[i][j][v] += (double)acc[i][j][v];
@@ -1184,7 +1184,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
 #pragma unroll
                     for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
             }
-            if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 2>(acc, wide);
+            if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 2, 2>(acc, wide);
         }
         TO_GLOBAL T* C = gptr<T>(d->c) + batch * d->cb;
         float mx = 0.0f;
@@ -1213,13 +1213,15 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
 // 128x128 output tiles: each wave owns a 64x64 quarter as 4x4 MFMA tiles; K advances by
 // BK (16 for double, 32 for float) with 16-byte global loads, and the next K slab is
 // fetched into registers while the current one is multiplied out of LDS.
-template <typename T, bool SCALED, bool WIDE>
+template <typename T, bool SCALED, bool WIDE, int BN>
 __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ descs, int ndesc) {
-    constexpr int BM = 128, BN = 128;
+    constexpr int BM = 128;
+    constexpr int NJ = BN / 32;                      // 16-column MFMA tiles per wave
     constexpr int VE = 16 / sizeof(T);              // elements per 16-byte load
     constexpr int BK = sizeof(T) == 8 ? 16 : 32;
     constexpr int VPR = BK / VE;                     // 16-byte vectors per tile row
-    constexpr int NV = BM * VPR / kBlock;            // vectors per thread per operand (4)
+    constexpr int NV = BM * VPR / kBlock;            // A vectors per thread (4)
+    constexpr int NVB = BN * VPR / kBlock;           // B vectors per thread (4 or 2)
     constexpr int LDK = BK + VE;                     // LDS row pitch: k contiguous, 16-byte pad
     __shared__ __align__(16) T As[BM][LDK];
     __shared__ __align__(16) T Bs[BN][LDK];
@@ -1229,7 +1231,7 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int64_t K = 1ll << d->klog;
-    const int64_t tiles_m = 1ll << (d->mlog - 7), tiles_n = 1ll << (d->nlog - 7);
+    const int64_t tiles_m = 1ll << (d->mlog - 7), tiles_n = 1ll << (d->nlog - (BN == 128 ? 7 : 6));
     const int64_t per_batch = tiles_m * tiles_n;
     const int64_t ntiles = (int64_t)d->nbatch * per_batch;
     int vrow[NV], vk[NV];
@@ -1251,55 +1253,54 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
             const OpScale x = op_scale(d->as, d->asb, batch), y = op_scale(d->bs, d->bsb, batch);
             sa = T(x.f); ea = x.e; sb = T(y.f); eb = y.e;
         }
-        V acc[4][4];
-        double wide[4][4][4];
+        V acc[4][NJ];
+        double wide[4][NJ][4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < NJ; ++j)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     acc[i][j][v] = T(0);
                     wide[i][j][v] = 0.0;
                 }
         int64_t step = 0;
-        L ra[NV], rb[NV];
+        L ra[NV], rb[NVB];
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            ra[i] = *reinterpret_cast<const TO_GLOBAL L*>(A + vrow[i] * K + vk[i]);
-            rb[i] = *reinterpret_cast<const TO_GLOBAL L*>(B + vrow[i] * K + vk[i]);
-        }
+        for (int i = 0; i < NV; ++i) ra[i] = *reinterpret_cast<const TO_GLOBAL L*>(A + vrow[i] * K + vk[i]);
+#pragma unroll
+        for (int i = 0; i < NVB; ++i) rb[i] = *reinterpret_cast<const TO_GLOBAL L*>(B + vrow[i] * K + vk[i]);
         for (int64_t k0 = 0; k0 < K; k0 += BK, ++step) {
             // rows keep k contiguous, so each 16-byte slab goes to LDS in one store; the
             // operands' power-of-two scales are applied to the outputs instead (exact)
             __syncthreads();
 #pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                *reinterpret_cast<L*>(&As[vrow[i]][vk[i]]) = ra[i];
-                *reinterpret_cast<L*>(&Bs[vrow[i]][vk[i]]) = rb[i];
-            }
+            for (int i = 0; i < NV; ++i) *reinterpret_cast<L*>(&As[vrow[i]][vk[i]]) = ra[i];
+#pragma unroll
+            for (int i = 0; i < NVB; ++i) *reinterpret_cast<L*>(&Bs[vrow[i]][vk[i]]) = rb[i];
             __syncthreads();
             if (k0 + BK < K) {
 #pragma unroll
-                for (int i = 0; i < NV; ++i) {
+                for (int i = 0; i < NV; ++i)
                     ra[i] = *reinterpret_cast<const TO_GLOBAL L*>(A + vrow[i] * K + k0 + BK + vk[i]);
+#pragma unroll
+                for (int i = 0; i < NVB; ++i)
                     rb[i] = *reinterpret_cast<const TO_GLOBAL L*>(B + vrow[i] * K + k0 + BK + vk[i]);
-                }
             }
 #pragma unroll
             for (int kk = 0; kk < BK / 4; ++kk) {
                 const int kr = kk * 4 + (lane >> 4);
-                T a[4], b[4];
+                T a[4], b[NJ];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) a[i] = As[wm * 64 + i * 16 + (lane & 15)][kr];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) b[j] = Bs[wn * 64 + j * 16 + (lane & 15)][kr];
+                for (int j = 0; j < NJ; ++j) b[j] = Bs[wn * (BN / 2) + j * 16 + (lane & 15)][kr];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
+                    for (int j = 0; j < NJ; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
             }
-            if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 4>(acc, wide);
+            if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 4, NJ>(acc, wide);
         }
 
         TO_GLOBAL T* C = gptr<T>(d->c) + batch * d->cb;
@@ -1308,8 +1309,8 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int64_t n = tn * BN + wn * 64 + j * 16 + (lane & 15);
+            for (int j = 0; j < NJ; ++j) {
+                const int64_t n = tn * BN + wn * (BN / 2) + j * 16 + (lane & 15);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int64_t m = tm * BM + wm * 64 + i * 16 + Mfma<T>::row(lane, v);
@@ -1327,10 +1328,11 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
     }
 }
 
-// narrow accumulation fits two waves per SIMD; the wide form's double accumulators do not
+// 128x128 tiles with float accumulators, and 128x64 tiles for the wide (double) form,
+// both at two waves per SIMD
 template <typename T, bool SCALED, bool WIDE>
-__global__ __launch_bounds__(kBlock, WIDE ? 1 : 2) void k_gemm128(const to_gemm_desc* __restrict__ descs, int ndesc) {
-    gemm128_body<T, SCALED, WIDE>(descs, ndesc);
+__global__ __launch_bounds__(kBlock, 2) void k_gemm128(const to_gemm_desc* __restrict__ descs, int ndesc) {
+    gemm128_body<T, SCALED, WIDE, WIDE ? 64 : 128>(descs, ndesc);
 }
 
 // ---------------------------------------------------------------------------------

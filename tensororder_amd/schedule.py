@@ -235,10 +235,11 @@ def choose_variant(g, nb, esize):
     fx, fy, s = g["fx"], g["fy"], g["s"]
     K = 1 << s
     if USE_GEMM and fx >= GEMM_MIN_FREE and fy >= GEMM_MIN_FREE and s >= GEMM_MIN_SHARED:
-        big = GEMM_TILES == 128 and fx >= 7 and fy >= 7 and s >= (4 if esize == 8 else 5)
-        tb = 7 if big else 6
-        tiles = nb << (fx - tb + fy - tb)
         wide = esize == 4 and s >= WIDE_ACCUM_MIN_SHARED
+        # 128-row tiles: 128 columns with float accumulators, 64 with double (wide)
+        tn = 6 if wide else 7
+        big = GEMM_TILES == 128 and fx >= 7 and fy >= tn and s >= (4 if esize == 8 else 5)
+        tiles = nb << ((fx - 7 + fy - tn) if big else (fx - 6 + fy - 6))
         return dict(kind="gemm", tile=(1 if big else 0) | (2 if wide else 0),
                     nblocks=int(min(tiles, MAX_GEMM_BLOCKS)),
                     perm_x=not _canonical(g["xf"], g["xs"]), perm_y=not _canonical(g["yf"], g["ys"]))
