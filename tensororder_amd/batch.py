@@ -18,7 +18,7 @@ from . import _lib, schedule
 from ._lib import Launch
 
 ALIGN = schedule.ALIGN
-KIND_ORDER = {"permute": 0, "stream": 1, "splitk": 1, "gemm": 1, "finish": 2}
+KIND_ORDER = schedule.KIND_ORDER
 
 
 class BatchProgram:

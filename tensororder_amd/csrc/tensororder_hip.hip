@@ -1213,14 +1213,18 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
 // 128x128 output tiles: each wave owns a 64x64 quarter as 4x4 MFMA tiles; K advances by
 // BK (16 for double, 32 for float) with 16-byte global loads, and the next K slab is
 // fetched into registers while the current one is multiplied out of LDS.
+#ifndef GEMM_F32_BK
+#define GEMM_F32_BK 64
+#endif
+
 template <typename T, bool SCALED, bool WIDE, int BN>
 __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ descs, int ndesc) {
     constexpr int BM = 128;
     constexpr int NJ = BN / 32;                      // 16-column MFMA tiles per wave
     constexpr int VE = 16 / sizeof(T);              // elements per 16-byte load
-    constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+    constexpr int BK = sizeof(T) == 8 ? 16 : GEMM_F32_BK;
     constexpr int VPR = BK / VE;                     // 16-byte vectors per tile row
-    constexpr int NV = BM * VPR / kBlock;            // A vectors per thread (4)
+    constexpr int NV = BM * VPR / kBlock;            // A vectors per thread
     constexpr int NVB = BN * VPR / kBlock;           // B vectors per thread (4 or 2)
     constexpr int LDK = BK + VE;                     // LDS row pitch: k contiguous, 16-byte pad
     __shared__ __align__(16) T As[BM][LDK];

@@ -311,12 +311,13 @@ class HipAPI(BaseTensorAPI):
                 raise OutOfMemoryError
             W = max(1, W // 2)
 
-    def _compile(self, tree, tables, num_groups, total_slices):
-        """Compile and upload a plan; returns an ``Executable``."""
+    def _compile(self, tree, tables, num_groups, total_slices, group_levels=True):
+        """Compile and upload a plan; returns an ``Executable``.  ``group_levels=False``
+        keeps one launch per join (per-join timing)."""
         nodes = schedule.analyse(tree, tables, self._regroup)
         W = self.choose_batch(nodes, tables, total_slices)
         prog = schedule.compile_program(tree, tables, num_groups, self._entry_type, W, self._alloc,
-                                        regroup=self._regroup)
+                                        regroup=self._regroup, group_levels=group_levels)
         self._upload(prog.table, prog.table_host.tobytes())
         self._upload(prog.coef, prog.coef_host.tobytes())
         self._upload(prog.blob, prog.blob_host.tobytes())

@@ -28,6 +28,8 @@ from ._lib import JoinDesc, LeafDesc, Launch
 
 ALIGN = 256
 STREAM_MAX_S = 12
+# order of a join's launches inside a group: operand permutes, products, split-K finishing
+KIND_ORDER = {"permute": 0, "stream": 1, "splitk": 1, "gemm": 1, "finish": 2}
 LDS_LIMIT = 64 * 1024
 MAX_STREAM_BLOCKS = 4096
 
@@ -433,9 +435,10 @@ def _pos_array(vals):
     return arr
 
 
-def _temporaries(arena, n, nodes, W, esize, partial):
+def _temporaries(arena, n, nodes, W, esize, partial, keep=False):
     """Scratch buffers a join needs while it runs (split-K partial sums, permuted GEMM
-    operands); allocated after the output, released before the next join."""
+    operands); allocated after the output and released at once, or returned for the
+    caller to release (``keep``) when other joins run beside it."""
     v = n.geom["var"]
     blocks = []
     if v["kind"] == "splitk":
@@ -448,38 +451,87 @@ def _temporaries(arena, n, nodes, W, esize, partial):
                 nb = W if (n.batched and op.batched) else 1
                 blocks.append((key, arena.alloc(nb * (1 << op.rank) * esize)))
     partial[n.idx] = dict(blocks)
+    if keep:
+        return [blk for _, blk in blocks]
     for _, blk in blocks:
         arena.release(blk)
+    return []
 
 
-def plan_memory(nodes, tables, W, esize):
-    """Arena placement for batch W.  Returns (arena peak bytes, scratch placements)."""
+def join_levels(nodes, batched):
+    """Joins of one phase (``batched`` False: the prologue, True: the wave) grouped by
+    level: a join's level is one more than the highest level among its children of the
+    same phase (leaves and the other phase's tensors count 0).  Joins of one level are
+    independent, so each level can run as grouped launches."""
+    level, out = {}, []
+    for n in nodes:
+        if n.is_leaf or n.batched != batched:
+            continue
+        lv = 1
+        for c in (n.left, n.right):
+            if c in level:
+                lv = max(lv, level[c] + 1)
+        level[n.idx] = lv
+        while len(out) < lv:
+            out.append([])
+        out[lv - 1].append(n.idx)
+    return out
+
+
+def plan_memory(nodes, tables, W, esize, order="post"):
+    """Arena placement for batch W when joins run one at a time in post-order
+    (``order="post"``) or level by level, all joins of a level at once (``"level"``).
+    Returns (arena peak bytes, scratch placements)."""
     arena = _Arena()
     partial = {}
-    # prologue: constant joins in post-order
-    for n in nodes:
-        if n.is_leaf or n.batched:
-            continue
-        n.buf = arena.alloc((1 << n.rank) * esize)
-        _temporaries(arena, n, nodes, 1, esize, partial)
-        for c in (n.left, n.right):
-            ch = nodes[c]
-            if not ch.is_leaf:
-                arena.release(ch.buf)
+
+    def run(group, nb, release_child):
+        # outputs and scratch of every join of the group coexist with all of its inputs
+        temps = []
+        for i in group:
+            n = nodes[i]
+            n.buf = arena.alloc(nb * (1 << n.rank) * esize)
+            temps += _temporaries(arena, n, nodes, nb, esize, partial, keep=True)
+        for blk in temps:
+            arena.release(blk)
+        for i in group:
+            n = nodes[i]
+            for c in (n.left, n.right):
+                if release_child(nodes[c]):
+                    arena.release(nodes[c].buf)
+
+    def groups(batched):
+        if order == "level":
+            return join_levels(nodes, batched)
+        return [[n.idx] for n in nodes if not n.is_leaf and n.batched == batched]
+
+    # prologue: slice-invariant joins; their outputs read by the wave stay allocated
+    for grp in groups(False):
+        run(grp, 1, lambda ch: not ch.is_leaf and not ch.batched)
     # wave: batched leaves are gathered together at wave start
     for n in nodes:
         if n.is_leaf and n.batched:
             n.buf = arena.alloc(W * (1 << n.rank) * esize)
-    for n in nodes:
-        if n.is_leaf or not n.batched:
-            continue
-        n.buf = arena.alloc(W * (1 << n.rank) * esize)
-        _temporaries(arena, n, nodes, W, esize, partial)
-        for c in (n.left, n.right):
-            ch = nodes[c]
-            if ch.batched:
-                arena.release(ch.buf)
+    for grp in groups(True):
+        run(grp, W, lambda ch: ch.batched)
     return arena.peak, partial
+
+
+# level-grouped execution is used when its arena peak stays within this factor of the
+# post-order peak (TENSORORDER_LEVEL_GROUPS=0 turns it off)
+LEVEL_GROUPS = os.environ.get("TENSORORDER_LEVEL_GROUPS", "1") != "0"
+LEVEL_PEAK_FACTOR = 1.10
+
+
+def choose_order(nodes, tables, W, esize):
+    """(order, arena peak) for a plan whose joins carry their variants."""
+    peak_post, _ = plan_memory(nodes, tables, W, esize, "post")
+    if not LEVEL_GROUPS:
+        return "post", peak_post
+    peak_level, _ = plan_memory(nodes, tables, W, esize, "level")
+    if peak_level <= LEVEL_PEAK_FACTOR * peak_post:
+        return "level", peak_level
+    return "post", peak_post
 
 
 def analyse(tree, tables, regroup=True):
@@ -509,8 +561,7 @@ def peak_bytes(nodes, tables, W, entry_type):
             g = join_geometry(nodes[n.left].rank, nodes[n.right].rank, n.lmap, n.rmap)
             g["var"] = choose_variant(g, nb, esize)
             n.geom = g
-    peak, _ = plan_memory(nodes, tables, W, esize)
-    return peak
+    return choose_order(nodes, tables, W, esize)[1]
 
 
 def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
@@ -584,11 +635,14 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
     return out
 
 
-def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regroup=True):
+def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regroup=True, group_levels=True):
     """Build a Program for batch size W.
 
     ``device_alloc(nbytes)`` returns an object with ``.data_ptr()`` (device memory that
-    stays alive as long as the Program).  ``tables`` maps tensor id -> LeafTable."""
+    stays alive as long as the Program).  ``tables`` maps tensor id -> LeafTable.  With
+    ``group_levels`` the joins run level by level as grouped launches (one per kernel
+    variant and level) when that keeps the arena within ``LEVEL_PEAK_FACTOR`` of the
+    post-order peak; otherwise, and for per-join timing, one launch per join."""
     esize = 8 if entry_type == _lib.TO_F64 else 4
     scaled = entry_type == _lib.TO_F32S
     fdtype = np.float64 if esize == 8 else np.float32
@@ -603,8 +657,10 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
             g = join_geometry(nodes[n.left].rank, nodes[n.right].rank, n.lmap, n.rmap)
             g["var"] = choose_variant(g, nb, esize)
             n.geom = g
-    peak, partial = plan_memory(nodes, tables, W, esize)
+    order = choose_order(nodes, tables, W, esize)[0] if group_levels else "post"
+    peak, partial = plan_memory(nodes, tables, W, esize, order)
     prog.arena_bytes = peak
+    prog.order = order
 
     # ---- leaf tables (host) -> one device buffer; the scaled executor's leaf factors
     # multiply the count (prog.leaf_factor, mantissa and exponent)
@@ -705,14 +761,48 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
             phase_list.append((kind, Launch(op=op, variant=variant, count=1, grid=nblk, lds=lds), off, n.idx,
                                work))
 
+    def level_launches(level, phase_list):
+        # every join of one level, grouped by kernel variant; permutes before products,
+        # products before split-K finishing
+        groups = {}
+        for i in level:
+            n = nodes[i]
+            for kind, op, variant, lds, st, nblk in join_records(n, nodes, W, esize, scaled, abase,
+                                                                   partial.get(n.idx, {}), recp):
+                if kind == "permute":
+                    work = (2 * esize * (int(st.nbatch) << int(st.rank)), 0)
+                elif kind in ("stream", "splitk", "gemm"):
+                    work = join_traffic(n, nodes, W, esize)
+                else:
+                    work = (0, 0)
+                key = (KIND_ORDER[kind], op, variant)
+                groups.setdefault(key, []).append((kind, st, nblk, lds, n.idx, work))
+        for key in sorted(groups):
+            grp = groups[key]
+            block0, lds, wb, wf, structs = 0, 0, 0, 0, []
+            for kind, st, nblk, l, idx, work in grp:
+                st.block0 = block0
+                block0 += nblk
+                lds = max(lds, l)
+                wb += work[0]
+                wf += work[1]
+                structs.append(st)
+            off = put(structs)
+            phase_list.append((grp[0][0], Launch(op=key[1], variant=key[2], count=len(structs), grid=block0,
+                                                 lds=lds), off, grp[0][4] if len(grp) == 1 else None, (wb, wf)))
+
     pro, wave = [], []
     if scaled and prog.rec_ranges[1] > prog.rec_ranges[0]:
         a, b = prog.rec_ranges[0], prog.rec_ranges[1]
         pro.append(("clear", Launch(op=_lib.TO_OP_SCALE_CLEAR, grid=max(1, min(1024, -(-(b - a) // 256))),
                                     aux0=rbase + a * rec_size, iaux0=b - a), None, None, (0, 0)))
-    for n in nodes:
-        if not n.is_leaf and not n.batched:
-            join_launches(n, pro)
+    if order == "level":
+        for level in join_levels(nodes, False):
+            level_launches(level, pro)
+    else:
+        for n in nodes:
+            if not n.is_leaf and not n.batched:
+                join_launches(n, pro)
     root = nodes[-1]
     prog.root_const = not root.batched
     if scaled and prog.rec_ranges[2] > prog.rec_ranges[1]:
@@ -741,9 +831,13 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
         leaf_bytes = sum(esize * (W << ld.rank) for ld in leaf_descs)
         wave.append(("leaf", Launch(op=_lib.TO_OP_LEAF_GATHER, count=len(leaf_descs), grid=block0,
                                     aux3=wave_params), off, None, (leaf_bytes, 0)))
-    for n in nodes:
-        if not n.is_leaf and n.batched:
-            join_launches(n, wave)
+    if order == "level":
+        for level in join_levels(nodes, True):
+            level_launches(level, wave)
+    else:
+        for n in nodes:
+            if not n.is_leaf and n.batched:
+                join_launches(n, wave)
     root_launch = Launch(op=_lib.TO_OP_ROOT_REDUCE, grid=1, aux0=root.ptr,
                          aux1=recp(root) if scaled else 0, aux2=prog.accum.data_ptr(),
                          aux3=pro_params if prog.root_const else wave_params,

@@ -50,7 +50,7 @@ def timed_records(api, plan, wave_flops=WAVE_FLOPS):
     api._max_batch = max(1, min(saved, 1 << int(math.log2(cap))))
     try:
         W = api.choose_batch(nodes, tables, total)
-        exe = api._compile(tree, tables, groups, min(total, W))
+        exe = api._compile(tree, tables, groups, min(total, W), group_levels=False)
     finally:
         api._max_batch = saved
     try:

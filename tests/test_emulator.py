@@ -100,43 +100,44 @@ def test_geometry_is_numpy_order():
     np.testing.assert_allclose(out, want, rtol=1e-13)
 
 
-def test_memory_plan_never_overlaps_live_buffers():
+@pytest.mark.parametrize("order", ["post", "level"])
+def test_memory_plan_never_overlaps_live_buffers(order):
+    # replay the execution groups (one join, or every join of a level): the outputs and
+    # scratch buffers of a group must not overlap each other nor any buffer that is still
+    # needed (an operand of this group or of a later one)
     doc = json.load(open(os.path.join(HERE, "golden", "cnf40_factor_s6.json")))
     plan = plan_io.plan_from_dict(doc)
     tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
     nodes = schedule.analyse(tree, tables)
+    consumer = {}
+    for n in nodes:
+        if not n.is_leaf:
+            for c in (n.left, n.right):
+                consumer[c] = n.idx
     for W in (1, 5, 64):
         schedule.peak_bytes(nodes, tables, W, _lib.TO_F64)
-        live = {}
-        for n in nodes:
-            if n.is_leaf and not n.batched:
-                continue
-            if n.buf is None:
-                continue
-        # replay the wave order and check that each join's output does not overlap its
-        # operands or any other still-needed buffer
-        needed = {}
-        for n in nodes:
-            if not n.is_leaf:
-                for c in (n.left, n.right):
-                    needed[c] = n.idx
-        order = [n for n in nodes if not n.is_leaf and not n.batched] + \
-                [n for n in nodes if n.is_leaf and n.batched] + \
-                [n for n in nodes if not n.is_leaf and n.batched]
+        peak, scratch = schedule.plan_memory(nodes, tables, W, 8, order)
+        if order == "level":
+            groups = schedule.join_levels(nodes, False) + [None] + schedule.join_levels(nodes, True)
+        else:
+            groups = [[n.idx] for n in nodes if not n.is_leaf and not n.batched] + [None] + \
+                     [[n.idx] for n in nodes if not n.is_leaf and n.batched]
         done = set()
-        for n in order:
-            if n.buf is None:
+        for grp in groups:
+            if grp is None:  # wave start: batched leaves are gathered
+                done |= {n.idx for n in nodes if n.is_leaf and n.batched}
                 continue
-            s0, z0 = n.buf
-            for m in nodes:
-                if m.idx == n.idx or m.buf is None or m.idx not in done:
-                    continue
-                consumer = needed.get(m.idx)
-                still_needed = consumer is not None and consumer not in done
-                if still_needed:
-                    s1, z1 = m.buf
-                    assert s0 + z0 <= s1 or s1 + z1 <= s0, (n.idx, m.idx)
-            done.add(n.idx)
+            fresh = [nodes[i].buf for i in grp] + [blk for i in grp for blk in scratch.get(i, {}).values()]
+            for k, (s0, z0) in enumerate(fresh):
+                assert s0 + z0 <= peak
+                for s1, z1 in fresh[k + 1:]:
+                    assert s0 + z0 <= s1 or s1 + z1 <= s0, ("group overlap", grp)
+                for m in done:
+                    if nodes[m].buf is None or consumer.get(m) is None or consumer[m] in done:
+                        continue
+                    s1, z1 = nodes[m].buf
+                    assert s0 + z0 <= s1 or s1 + z1 <= s0, ("live overlap", grp, m)
+            done |= set(grp)
 
 
 @pytest.mark.parametrize("et,tol", [(_lib.TO_F64, 1e-12), (_lib.TO_F32S, 1e-6)])
