@@ -1077,6 +1077,16 @@ __global__ __launch_bounds__(kBlock) void k_permute(const to_perm_desc desc) {
 // i + 16k, B[k][j] in lane j + 16k; the result D[i][j] is in lane j + 16(i/4), element
 // i%4 for f32, and in lane j + 16(i%4), element i/4 for f64 (Mfma<T>::row).
 
+// Workgroups are dealt to the 8 XCDs round-robin, and each XCD has its own L2.  Tile
+// index t (block order) is mapped so that the tiles one XCD runs are consecutive in
+// (batch, tm, tn) order and share their A panel (same tm) in that XCD's L2: the blocks
+// t = 8q + x run tile x * (ntiles / 8) + q.  A bijection of [0, ntiles) whenever ntiles
+// is a multiple of 8, the identity otherwise.
+__device__ __forceinline__ int64_t xcd_tile(int64_t t, int64_t ntiles) {
+    if (ntiles & 7) return t;
+    return (t & 7) * (ntiles >> 3) + (t >> 3);
+}
+
 template <typename T>
 struct Mfma;
 
@@ -1133,7 +1143,8 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
     const int64_t per_batch = tiles_m * tiles_n;
     const int64_t ntiles = (int64_t)d->nbatch * per_batch;
     const int lrow = tid >> 2, lk = (tid & 3) * 4;
-    for (int64_t t = blockIdx.x - d->block0; t < ntiles; t += d->nblocks) {
+    for (int64_t t0 = blockIdx.x - d->block0; t0 < ntiles; t0 += d->nblocks) {
+        const int64_t t = xcd_tile(t0, ntiles);
         const int64_t batch = t / per_batch;
         const int64_t rem = t - batch * per_batch;
         const int64_t tm = rem / tiles_n, tn = rem - (rem / tiles_n) * tiles_n;
@@ -1213,16 +1224,12 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
 // 128x128 output tiles: each wave owns a 64x64 quarter as 4x4 MFMA tiles; K advances by
 // BK (16 for double, 32 for float) with 16-byte global loads, and the next K slab is
 // fetched into registers while the current one is multiplied out of LDS.
-#ifndef GEMM_F32_BK
-#define GEMM_F32_BK 64
-#endif
-
 template <typename T, bool SCALED, bool WIDE, int BN>
 __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ descs, int ndesc) {
     constexpr int BM = 128;
     constexpr int NJ = BN / 32;                      // 16-column MFMA tiles per wave
     constexpr int VE = 16 / sizeof(T);              // elements per 16-byte load
-    constexpr int BK = sizeof(T) == 8 ? 16 : GEMM_F32_BK;
+    constexpr int BK = sizeof(T) == 8 ? 16 : 32;     // the host requires K >= BK
     constexpr int VPR = BK / VE;                     // 16-byte vectors per tile row
     constexpr int NV = BM * VPR / kBlock;            // A vectors per thread
     constexpr int NVB = BN * VPR / kBlock;           // B vectors per thread (4 or 2)
@@ -1245,7 +1252,8 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
         vrow[i] = v / VPR;
         vk[i] = (v % VPR) * VE;
     }
-    for (int64_t t = blockIdx.x - d->block0; t < ntiles; t += d->nblocks) {
+    for (int64_t t0 = blockIdx.x - d->block0; t0 < ntiles; t0 += d->nblocks) {
+        const int64_t t = xcd_tile(t0, ntiles);
         const int64_t batch = t / per_batch;
         const int64_t rem = t - batch * per_batch;
         const int64_t tm = rem / tiles_n, tn = rem - (rem / tiles_n) * tiles_n;

@@ -2,6 +2,7 @@
 reproduce the oracle: bit placement, batching, slice-row gather, memory plan, split-K and
 the (mantissa, exponent) slice sum, in both entry types."""
 
+import ctypes
 import json
 import os
 
@@ -157,3 +158,44 @@ def test_batch_of_uneven_instances_emulated(et, tol):
     assert prog.stats["steps"] < prog.stats["joins"]
     for g, w in zip(got, wants):
         assert g == pytest.approx(w, rel=tol)
+
+
+def _gemm_descs(prog):
+    """(variant, GemmDesc) of every MFMA product launch of a compiled program."""
+    out = []
+    for L in list(prog.prologue) + list(prog.wave):
+        if L.op != _lib.TO_OP_GEMM:
+            continue
+        base = L.descs - prog.blob.data_ptr()
+        for i in range(L.count):
+            off = base + i * ctypes.sizeof(_lib.GemmDesc)
+            out.append((L.variant, _lib.GemmDesc.from_buffer_copy(prog.blob_host.tobytes()[off:off + ctypes.sizeof(_lib.GemmDesc)])))
+    return out
+
+
+@pytest.mark.parametrize("et", [_lib.TO_F64, _lib.TO_F32S])
+def test_gemm_descriptors_fit_their_tiles(et):
+    # every product the schedule hands to the MFMA kernels satisfies the tile shape the
+    # variant assumes: rows/columns >= the tile edge, shared range >= one K slab
+    cases = []
+    for path in plan_fixture_paths():
+        doc = json.load(open(path))
+        if doc["meta"]["maxrank"] <= 20:
+            cases.append(plan_io.plan_from_dict(doc))
+    for ra, rb, s in ((13, 13, 5), (14, 13, 6), (16, 14, 4), (18, 17, 11), (12, 12, 6)):
+        cases.append(_two_tensor_plan(ra, rb, s, seed=1))
+    f32 = et == _lib.TO_F32S
+    seen = 0
+    for plan in cases:
+        tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
+        heap = emulator.Heap()
+        prog = schedule.compile_program(tree, tables, G, et, min(total, 4), heap.alloc)
+        for variant, gd in _gemm_descs(prog):
+            seen += 1
+            if variant & 1:
+                assert gd.mlog >= 7 and gd.nlog >= (6 if variant & 2 else 7)
+                assert gd.klog >= (5 if f32 else 4)  # K slab of 32 floats / 16 doubles
+            else:
+                assert gd.mlog >= 6 and gd.nlog >= 6 and gd.klog >= 4
+            assert not (variant & 2) or f32
+    assert seen > 0
