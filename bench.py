@@ -50,7 +50,9 @@ def kernel_symbol(L, scaled):
     S = "true" if scaled else "false"
     op, v = L.op, L.variant
     if op == _lib.TO_OP_GEMM:
-        return "%s<%s, %s, %s>" % ("k_gemm128" if v & 1 else "k_gemm", T, S, "true" if v & 2 else "false")
+        if v & 1:
+            return "k_gemm128<%s, %s, %s, %s>" % (T, S, "true" if v & 2 else "false", "true" if v & 4 else "false")
+        return "k_gemm<%s, %s, %s>" % (T, S, "true" if v & 2 else "false")
     if op == _lib.TO_OP_JOIN_STREAM:
         return "k_join_stream<%s, %s, %d>" % (T, S, v)
     if op == _lib.TO_OP_PERMUTE:

@@ -66,7 +66,9 @@ extern "C" {
 #define TO_OP_GEMM 10        /* descs: to_gemm_desc[count]; variant bit 0: 128-row output
                                 tiles (mlog >= 7), 128 columns (nlog >= 7) or, with bit 1,
                                 64 columns (nlog >= 6), instead of 64x64 (>= 6); bit 1: float
-                                partial sums accumulated in double every 4 K-steps */
+                                partial sums accumulated in double every 4 K-steps; bit 2
+                                (with bit 0): the product is formed transposed, so a float
+                                output with ldcn == 1 is written with 16-byte stores */
 #define TO_OP_ROOT_STORE 11  /* descs: to_root_desc[count]: one instance's root -> its accumulator */
 #define TO_OP_BATCH_BEGIN 12 /* aux2: to_accum[iaux0] zeroed; aux3: int64 start time (device clock) */
 

@@ -1224,7 +1224,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
 // 128x128 output tiles: each wave owns a 64x64 quarter as 4x4 MFMA tiles; K advances by
 // BK (16 for double, 32 for float) with 16-byte global loads, and the next K slab is
 // fetched into registers while the current one is multiplied out of LDS.
-template <typename T, bool SCALED, bool WIDE, int BN>
+template <typename T, bool SCALED, bool WIDE, int BN, bool TR>
 __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ descs, int ndesc) {
     constexpr int BM = 128;
     constexpr int NJ = BN / 32;                      // 16-column MFMA tiles per wave
@@ -1310,25 +1310,44 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int j = 0; j < NJ; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
+                    for (int j = 0; j < NJ; ++j)
+                        acc[i][j] = TR ? Mfma<T>::op(b[j], a[i], acc[i][j]) : Mfma<T>::op(a[i], b[j], acc[i][j]);
             }
             if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 4, NJ>(acc, wide);
         }
 
+        // Result element v of a lane: without TR, row (m) Mfma::row(lane, v) of column
+        // lane & 15; with TR (product computed transposed), column (n) Mfma::row(lane, v)
+        // of row lane & 15.  For float the 4 elements are consecutive, so when they are
+        // adjacent in C (rows with ldcm == 1, columns with TR and ldcn == 1) one 16-byte
+        // store writes them.
         TO_GLOBAL T* C = gptr<T>(d->c) + batch * d->cb;
         float mx = 0.0f;
         const T sab = sa * sb;
+        const bool vec_out = sizeof(T) == 4 && (TR ? d->ldcn == 1 : d->ldcm == 1);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
 #pragma unroll
             for (int j = 0; j < NJ; ++j) {
-                const int64_t n = tn * BN + wn * (BN / 2) + j * 16 + (lane & 15);
+                T out[4];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int64_t m = tm * BM + wm * 64 + i * 16 + Mfma<T>::row(lane, v);
-                    const T out = (kWideAccum<T, WIDE> ? T(wide[i][j][v]) : acc[i][j][v]) * sab;
-                    C[m * d->ldcm + n * d->ldcn] = out;
-                    if (SCALED) mx = fmaxf(mx, (float)absval(out));
+                    out[v] = (kWideAccum<T, WIDE> ? T(wide[i][j][v]) : acc[i][j][v]) * sab;
+                    if (SCALED) mx = fmaxf(mx, (float)absval(out[v]));
+                }
+                const int64_t mb = tm * BM + wm * 64 + i * 16, nb = tn * BN + wn * (BN / 2) + j * 16;
+                if (vec_out) {
+                    typedef T V4 __attribute__((ext_vector_type(4)));
+                    const int64_t m = TR ? mb + (lane & 15) : mb + Mfma<T>::row(lane, 0);
+                    const int64_t n = TR ? nb + Mfma<T>::row(lane, 0) : nb + (lane & 15);
+                    *reinterpret_cast<TO_GLOBAL V4*>(C + m * d->ldcm + n * d->ldcn) = V4{out[0], out[1], out[2], out[3]};
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int64_t m = TR ? mb + (lane & 15) : mb + Mfma<T>::row(lane, v);
+                        const int64_t n = TR ? nb + Mfma<T>::row(lane, v) : nb + (lane & 15);
+                        C[m * d->ldcm + n * d->ldcn] = out[v];
+                    }
                 }
             }
         }
@@ -1342,9 +1361,9 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
 
 // 128x128 tiles with float accumulators, and 128x64 tiles for the wide (double) form,
 // both at two waves per SIMD
-template <typename T, bool SCALED, bool WIDE>
+template <typename T, bool SCALED, bool WIDE, bool TR>
 __global__ __launch_bounds__(kBlock, 2) void k_gemm128(const to_gemm_desc* __restrict__ descs, int ndesc) {
-    gemm128_body<T, SCALED, WIDE, WIDE ? 64 : 128>(descs, ndesc);
+    gemm128_body<T, SCALED, WIDE, WIDE ? 64 : 128, TR>(descs, ndesc);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1433,10 +1452,16 @@ int launch_one(const to_launch& L, hipStream_t st) {
         case TO_OP_GEMM:
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
             if (L.variant == 1)
-                hipLaunchKernelGGL((k_gemm128<T, SCALED, false>), grid, block, 0, st,
+                hipLaunchKernelGGL((k_gemm128<T, SCALED, false, false>), grid, block, 0, st,
                                    reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
             else if (L.variant == 3)
-                hipLaunchKernelGGL((k_gemm128<T, SCALED, true>), grid, block, 0, st,
+                hipLaunchKernelGGL((k_gemm128<T, SCALED, true, false>), grid, block, 0, st,
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+            else if (L.variant == 5)
+                hipLaunchKernelGGL((k_gemm128<T, SCALED, false, true>), grid, block, 0, st,
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+            else if (L.variant == 7)
+                hipLaunchKernelGGL((k_gemm128<T, SCALED, true, true>), grid, block, 0, st,
                                    reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
             else if (L.variant == 0)
                 hipLaunchKernelGGL((k_gemm<T, SCALED, false>), grid, block, 0, st,

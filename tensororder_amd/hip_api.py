@@ -6,8 +6,9 @@ Mirrors the plugin interface the reference's ``execution.run`` drives
 * ``add_argument(key, value)``: ``entry_type`` (``float64`` = plain FP64 executor,
   ``float32`` = FP32 with per-tensor power-of-two scaling), ``thread_limit`` (accepted;
   host threads do no arithmetic here), ``max_batch``, ``mem_limit`` (bytes of device
-  memory the executor may use), ``graph`` (capture waves in a HIP graph); any other key
-  raises ``ValueError`` (``base_api.py:9-12``).
+  memory the executor may use), ``graph`` (capture waves in a HIP graph), ``regroup``,
+  ``autotune`` (time the launch shapes of new product/permute shapes before compiling);
+  any other key raises ``ValueError`` (``base_api.py:9-12``).
 * ``contract_sliced(plan, num_slice_limit)``: all slices of the plan, batched in waves on
   the device, summed in FP64 (mantissa, exponent) -- the override pattern of
   ``jax_apis.py:316-322``; returns a Python float.
@@ -213,6 +214,11 @@ class HipAPI(BaseTensorAPI):
         self._mem_limit = None
         self._use_graph = True
         self._regroup = True
+        self._autotune = False
+        # launch shapes tuned on this GPU model (autotune.py), used where a shape is known
+        from . import autotune
+
+        schedule.TUNED.update(autotune.load())
         self.last_result = None
         self.last_stats = {}
 
@@ -234,6 +240,8 @@ class HipAPI(BaseTensorAPI):
             self._use_graph = bool(value)
         elif key == "regroup":
             self._regroup = bool(value)
+        elif key == "autotune":
+            self._autotune = bool(value)
         else:
             super().add_argument(key, value)
 
@@ -316,6 +324,11 @@ class HipAPI(BaseTensorAPI):
         keeps one launch per join (per-join timing)."""
         nodes = schedule.analyse(tree, tables, self._regroup)
         W = self.choose_batch(nodes, tables, total_slices)
+        if self._autotune:
+            from . import autotune
+
+            if autotune.tune_plan(self, nodes, W, schedule.TUNED):
+                autotune.save(schedule.TUNED)
         prog = schedule.compile_program(tree, tables, num_groups, self._entry_type, W, self._alloc,
                                         regroup=self._regroup, group_levels=group_levels)
         self._upload(prog.table, prog.table_host.tobytes())

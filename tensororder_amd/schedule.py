@@ -232,6 +232,18 @@ def _canonical(free_pos, shared_pos):
     return list(shared_pos) == list(range(s)) and list(free_pos) == list(range(s, s + len(free_pos)))
 
 
+# Autotuned launch shapes (``autotune.py``): key -> choice; shapes not in it keep the rule.
+TUNED = {}
+
+
+def gemm_key(esize, mlog, nlog, klog, nb):
+    return "gemm/%d/%d/%d/%d/%d" % (esize, mlog, nlog, klog, int(math.log2(max(1, nb))))
+
+
+def perm_key(esize, rank, nb):
+    return "perm/%d/%d/%d" % (esize, rank, int(math.log2(max(1, nb))))
+
+
 def choose_variant(g, nb, esize):
     """Kernel variant and launch shape for a join with ``nb`` batch items."""
     fx, fy, s = g["fx"], g["fy"], g["s"]
@@ -241,9 +253,17 @@ def choose_variant(g, nb, esize):
         # 128-row tiles: 128 columns with float accumulators, 64 with double (wide)
         tn = 6 if wide else 7
         big = GEMM_TILES == 128 and fx >= 7 and fy >= tn and s >= (4 if esize == 8 else 5)
+        cap = MAX_GEMM_BLOCKS
+        tuned = TUNED.get(gemm_key(esize, fx, fy, s, nb))
+        if tuned is not None and bool(tuned["variant"] & 2) == wide and (big or not tuned["variant"] & 1):
+            big = bool(tuned["variant"] & 1)  # the tile shape; bit 2 follows the layout below
+            cap = tuned.get("block_cap") or MAX_GEMM_BLOCKS
         tiles = nb << ((fx - 7 + fy - tn) if big else (fx - 6 + fy - 6))
-        return dict(kind="gemm", tile=(1 if big else 0) | (2 if wide else 0),
-                    nblocks=int(min(tiles, MAX_GEMM_BLOCKS)),
+        # float outputs whose columns are contiguous (shy == 0) are written 16 bytes at a
+        # time from a transposed product
+        tr = big and esize == 4 and g["shy"] == 0
+        return dict(kind="gemm", tile=(1 if big else 0) | (2 if wide else 0) | (4 if tr else 0),
+                    nblocks=int(min(tiles, cap)),
                     perm_x=not _canonical(g["xf"], g["xs"]), perm_y=not _canonical(g["yf"], g["ys"]))
     yc = min(16, 1 << fy)
     stage = 0
@@ -374,9 +394,13 @@ def perm_desc(src, dst, rank, out_pos, nbatch, stride, esize):
             arr[i] = x
     ntiles = nbatch << len(rest_in)
     d.block0 = 0
-    # the chunk kernel holds 4 blocks per CU (36 KB of LDS each): one resident wave of
-    # blocks, each walking a contiguous range of tiles
-    d.nblocks = int(min(ntiles, 1024 if vec else 4096))
+    # the chunk kernel holds 4 blocks per CU (36 KB of LDS each): by rule one resident wave
+    # of blocks, each walking a contiguous range of tiles
+    cap = 1024 if vec else 4096
+    tuned = TUNED.get(perm_key(esize, rank, nbatch)) if vec else None
+    if tuned is not None:
+        cap = tuned["block_cap"]
+    d.nblocks = int(min(ntiles, cap))
     lds = ((esize << len(u_in)) + 15) // 16 * 16 + 2 * 256 * 8
     return d, d.nblocks, lds, 1 if vec else 0
 
