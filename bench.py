@@ -54,7 +54,11 @@ def kernel_symbol(L, scaled):
             return "k_gemm128<%s, %s, %s, %s>" % (T, S, "true" if v & 2 else "false", "true" if v & 4 else "false")
         return "k_gemm<%s, %s, %s>" % (T, S, "true" if v & 2 else "false")
     if op == _lib.TO_OP_JOIN_STREAM:
+        if v & _lib.TO_STREAM_VEC:
+            return "k_join_stream_vec<%s, %s, %d>" % (T, S, v & ~_lib.TO_STREAM_VEC)
         return "k_join_stream<%s, %s, %d>" % (T, S, v)
+    if op == _lib.TO_OP_JOIN_FINISH:
+        return "%s<%s, %s>" % ("k_join_finish_wide" if v == 1 else "k_join_finish", T, S)
     if op == _lib.TO_OP_PERMUTE:
         return "%s<%s>" % ("k_permute_vec_desc" if v == 1 else "k_permute_desc", T)
     names = {_lib.TO_OP_LEAF_GATHER: "k_leaf_gather", _lib.TO_OP_JOIN_SPLITK: "k_join_splitk",
