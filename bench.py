@@ -36,6 +36,7 @@ WORKLOADS = {
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0  # dense (no sparsity)
 
 
 def log(*a):
@@ -50,8 +51,13 @@ def kernel_symbol(L, scaled):
     S = "true" if scaled else "false"
     op, v = L.op, L.variant
     if op == _lib.TO_OP_GEMM:
+        wide, tr = "true" if v & 2 else "false", "true" if v & 4 else "false"
+        if v & 16:
+            return "k_gemm_x6<%s, %s, %s>" % (S, wide, tr)
+        if v & 8:
+            return "k_gemm128_db<%s, %s, %s, %s>" % (T, S, wide, tr)
         if v & 1:
-            return "k_gemm128<%s, %s, %s, %s>" % (T, S, "true" if v & 2 else "false", "true" if v & 4 else "false")
+            return "k_gemm128<%s, %s, %s, %s>" % (T, S, wide, tr)
         return "k_gemm<%s, %s, %s>" % (T, S, "true" if v & 2 else "false")
     if op == _lib.TO_OP_JOIN_STREAM:
         if v & _lib.TO_STREAM_VEC:
@@ -86,13 +92,19 @@ def measured_traffic(workload, name):
 def roofline_of(name, k, workload=None):
     """Roofline object of one kernel symbol from its summed launches of one step."""
     if k["flops"] and k["flops"] / max(k["bytes"], 1) > 20 and name.startswith("k_gemm"):
-        peak = FP32_PEAK_TFLOPS if "<float" in name else FP64_PEAK_TFLOPS
         achieved = k["flops"] / (k["ms"] * 1e-3) / 1e12 if k["ms"] > 0 else 0.0
-        return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": measured_traffic(workload, name), "kernel": name,
-                "launches_per_step": k["launches"],
-                "kernel_ms_per_step": k["ms"], "avg_launch_us": 1e3 * k["ms"] / max(k["launches"], 1),
-                "algorithmic_flops_per_step": k["flops"]}
+        out = {"bound": "mfma", "achieved": achieved, "unit": "TFLOP/s", "kernel": name}
+        if name.startswith("k_gemm_x6"):
+            # float products on the bf16 matrix cores: each float multiply-add is six bf16
+            # ones, so the float flops are priced against the dense bf16 peak / 6
+            peak = BF16_PEAK_TFLOPS / 6
+            out["peak_basis"] = "dense bf16 peak %.0f TFLOP/s / 6 bf16 products per float product" % BF16_PEAK_TFLOPS
+        else:
+            peak = FP32_PEAK_TFLOPS if "<float" in name else FP64_PEAK_TFLOPS
+        out.update({"peak": peak, "frac": achieved / peak, "traffic": measured_traffic(workload, name),
+                    "launches_per_step": k["launches"], "kernel_ms_per_step": k["ms"],
+                    "avg_launch_us": 1e3 * k["ms"] / max(k["launches"], 1), "algorithmic_flops_per_step": k["flops"]})
+        return out
     achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload, name), "kernel": name,

@@ -68,7 +68,13 @@ extern "C" {
                                 64 columns (nlog >= 6), instead of 64x64 (>= 6); bit 1: float
                                 partial sums accumulated in double every 4 K-steps; bit 2
                                 (with bit 0): the product is formed transposed, so a float
-                                output with ldcn == 1 is written with 16-byte stores */
+                                output with ldcn == 1 is written with 16-byte stores; bit 3
+                                (with bit 0): 128x128 tiles (nlog >= 7) at one block per CU,
+                                LDS double-buffered (variants 9, 11, 13, 15); bit 4 (float,
+                                with bit 0, 128x128 tiles): the product on the bf16 matrix
+                                cores, each operand split into three bf16 parts and six
+                                partial products summed (with bit 3: variants 25, 27, 29,
+                                31; with bit 1, klog >= 7) */
 #define TO_OP_ROOT_STORE 11  /* descs: to_root_desc[count]: one instance's root -> its accumulator */
 #define TO_OP_BATCH_BEGIN 12 /* aux2: to_accum[iaux0] zeroed; aux3: int64 start time (device clock) */
 

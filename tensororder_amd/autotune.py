@@ -55,10 +55,19 @@ def gemm_candidates(esize, mlog, nlog, klog, nb, wide):
     tiles64 = nb << (mlog - 6 + nlog - 6)
     for cap in (2048, 8192):
         out.append(((2 if wide else 0), int(min(tiles64, cap))))
-    if mlog >= 7 and nlog >= tn and klog >= (4 if esize == 8 else 5):
+    if mlog >= 7 and nlog >= tn and klog >= schedule.GEMM128_MIN_SHARED[(esize, wide)]:
         tiles128 = nb << (mlog - 7 + nlog - tn)
         for cap in (1024, 2048, 8192):
             out.append((1 | (2 if wide else 0), int(min(tiles128, cap))))
+        if nlog >= 7:
+            # 128x128 tiles at one block per CU (LDS double-buffered)
+            tiles128 = nb << (mlog - 7 + nlog - 7)
+            for cap in (256, 512, 1024, 8192):
+                out.append((1 | (2 if wide else 0) | 8, int(min(tiles128, cap))))
+            if esize == 4 and schedule.GEMM_X6 and (klog >= 7 or not wide):
+                # float products on the bf16 matrix cores (one block per CU)
+                for cap in (256, 512, 1024, 8192):
+                    out.append((1 | (2 if wide else 0) | 8 | 16, int(min(tiles128, cap))))
     return sorted(set(out))
 
 
@@ -122,7 +131,7 @@ def tune_gemm(api, mlog, nlog, klog, nb):
         results["%d/%d" % (variant, nblocks)] = _time(api, [L]) / nbt
     best = min(results, key=results.get)
     v, n = (int(x) for x in best.split("/"))
-    v &= 3  # the tile shape; the transposed-output bit is chosen by the output layout
+    v &= 27  # the tile shape; the transposed-output bit is chosen by the output layout
     # the block count scales with the batch: keep it relative to the full tile count
     return {"variant": v, "block_cap": n if nbt == nb else None, "ms_per_item": results[best]}, results
 

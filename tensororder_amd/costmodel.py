@@ -41,7 +41,7 @@ def join_features(n, nodes, W, esize):
     kind = v["kind"]
     perms = (int(v.get("perm_x", False)) + int(v.get("perm_y", False))) if kind == "gemm" else 0
     variant = {"stream": math.log2(v.get("yc", 1)) + (5 if v.get("vec") else 0),
-               "gemm": v.get("tile", 0)}.get(kind, 0)
+               "gemm": v.get("tile", 0) & 7}.get(kind, 0)
     return [X.rank, Y.rank, n.rank, g["s"], g["fx"], g["fy"], math.log2(max(f, 1)), math.log2(max(b, 1)),
             math.log2(nb), kind == "stream", kind == "splitk", kind == "gemm", esize / 8.0, perms,
             g["fx"] + g["fy"] + math.log2(nb), variant]

@@ -1087,6 +1087,35 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t t, int64_t ntiles) {
     return (t & 7) * (ntiles >> 3) + (t >> 3);
 }
 
+// Tile (tm, tn) of the rem-th tile of one batch item, in groups of 8 tile rows: the
+// tiles of a group are taken column by column (8 row tiles, then the next column), so the
+// blocks running at once on one XCD cover about an 8x8 patch and share 8 A and 8 B
+// panels in its L2, instead of one A panel and every B panel of a row.  A bijection of
+// [0, tiles_m * tiles_n) (both powers of two).
+__device__ __forceinline__ void grouped_tile(int64_t rem, int64_t tiles_m, int64_t tiles_n, int64_t& tm,
+                                             int64_t& tn) {
+    const int64_t g = tiles_m < 8 ? tiles_m : 8;
+    const int64_t span = g * tiles_n;
+    const int64_t group = rem / span, r = rem - group * span;
+    tm = group * g + (r & (g - 1));
+    tn = r / g;
+}
+
+// A block walks its tiles in order and keeps one running max |output| per wave for the
+// batch item of the current tile; it is published (one atomic per wave) when the batch
+// item changes and after the last tile, rather than once per tile.  `next` is the batch
+// item of the coming tile (-1 after the last); wave-uniform, as the cross-lane max needs.
+__device__ __forceinline__ void publish_max_on_change(uint64_t recs, int64_t& cur, int64_t next, float& mx,
+                                                      int lane) {
+    if (next == cur) return;
+    if (cur >= 0) {
+        const float w = wave_max(mx);
+        if (lane == 0) note_max(recs, cur, w);
+    }
+    cur = next;
+    mx = 0.0f;
+}
+
 template <typename T>
 struct Mfma;
 
@@ -1109,8 +1138,8 @@ struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) * 4 + v; }
 };
 
-// With WIDE, float products are summed by the MFMA over 4 K-steps (64 or 128 terms) and
-// those partial sums are accumulated in double, so a long shared range (K up to 2^15 here)
+// With WIDE, float products are summed by the MFMA over 64 (64x64 tiles) or 128 (128-row
+// tiles) terms and those partial sums are accumulated in double, so a long shared range (K up to 2^15 here)
 // does not grow the float rounding error with sqrt(K); it costs registers (occupancy), so
 // the host asks for it only for long shared ranges.
 template <typename T, bool WIDE>
@@ -1143,11 +1172,15 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
     const int64_t per_batch = tiles_m * tiles_n;
     const int64_t ntiles = (int64_t)d->nbatch * per_batch;
     const int lrow = tid >> 2, lk = (tid & 3) * 4;
+    float mx = 0.0f;        // running max |output| of the batch item mx_batch
+    int64_t mx_batch = -1;
     for (int64_t t0 = blockIdx.x - d->block0; t0 < ntiles; t0 += d->nblocks) {
         const int64_t t = xcd_tile(t0, ntiles);
         const int64_t batch = t / per_batch;
         const int64_t rem = t - batch * per_batch;
-        const int64_t tm = rem / tiles_n, tn = rem - (rem / tiles_n) * tiles_n;
+        int64_t tm, tn;
+        grouped_tile(rem, tiles_m, tiles_n, tm, tn);
+        if (SCALED) publish_max_on_change(d->cs, mx_batch, batch, mx, lane);
         const TO_GLOBAL T* A = gptr<const T>(d->a) + batch * d->ab + (tm * BM + lrow) * K + lk;
         const TO_GLOBAL T* B = gptr<const T>(d->b) + batch * d->bb + (tn * BN + lrow) * K + lk;
         T sa = T(1), sb = T(1);
@@ -1198,7 +1231,6 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
             if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 2, 2>(acc, wide);
         }
         TO_GLOBAL T* C = gptr<T>(d->c) + batch * d->cb;
-        float mx = 0.0f;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
 #pragma unroll
@@ -1213,29 +1245,31 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
                 }
             }
         }
-        if (SCALED) {
-            mx = wave_max(mx);
-            if (lane == 0) note_max(d->cs, batch, mx);
-            if (tid == 0 && tm == 0 && tn == 0) note_exp(d->cs, batch, ea + eb);
-        }
+        if (SCALED && tid == 0 && tm == 0 && tn == 0) note_exp(d->cs, batch, ea + eb);
     }
+    if (SCALED) publish_max_on_change(d->cs, mx_batch, -1, mx, lane);
 }
 
 // 128x128 output tiles: each wave owns a 64x64 quarter as 4x4 MFMA tiles; K advances by
 // BK (16 for double, 32 for float) with 16-byte global loads, and the next K slab is
-// fetched into registers while the current one is multiplied out of LDS.
-template <typename T, bool SCALED, bool WIDE, int BN, bool TR>
+// fetched into registers while the current one is multiplied out of LDS.  With DB the
+// LDS slabs are double-buffered: the next slab is stored while the current one is still
+// being read, so one barrier per K step suffices (the form for one block per CU, where no
+// second block covers a barrier wait).
+template <typename T, bool SCALED, bool WIDE, int BN, bool TR, bool DB>
 __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ descs, int ndesc) {
     constexpr int BM = 128;
+    constexpr int NBUF = DB ? 2 : 1;
     constexpr int NJ = BN / 32;                      // 16-column MFMA tiles per wave
     constexpr int VE = 16 / sizeof(T);              // elements per 16-byte load
     constexpr int BK = sizeof(T) == 8 ? 16 : 32;     // the host requires K >= BK
+    constexpr int FLUSH = 128 / BK;                  // K steps per float partial sum (128 terms)
     constexpr int VPR = BK / VE;                     // 16-byte vectors per tile row
     constexpr int NV = BM * VPR / kBlock;            // A vectors per thread
     constexpr int NVB = BN * VPR / kBlock;           // B vectors per thread (4 or 2)
     constexpr int LDK = BK + VE;                     // LDS row pitch: k contiguous, 16-byte pad
-    __shared__ __align__(16) T As[BM][LDK];
-    __shared__ __align__(16) T Bs[BN][LDK];
+    __shared__ __align__(16) T As[NBUF][BM][LDK];
+    __shared__ __align__(16) T Bs[NBUF][BN][LDK];
     typedef typename Mfma<T>::V V;
     typedef T L __attribute__((ext_vector_type(VE)));
     const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
@@ -1252,11 +1286,15 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
         vrow[i] = v / VPR;
         vk[i] = (v % VPR) * VE;
     }
+    float mx = 0.0f;        // running max |output| of the batch item mx_batch
+    int64_t mx_batch = -1;
     for (int64_t t0 = blockIdx.x - d->block0; t0 < ntiles; t0 += d->nblocks) {
         const int64_t t = xcd_tile(t0, ntiles);
         const int64_t batch = t / per_batch;
         const int64_t rem = t - batch * per_batch;
-        const int64_t tm = rem / tiles_n, tn = rem - (rem / tiles_n) * tiles_n;
+        int64_t tm, tn;
+        grouped_tile(rem, tiles_m, tiles_n, tm, tn);
+        if (SCALED) publish_max_on_change(d->cs, mx_batch, batch, mx, lane);
         const TO_GLOBAL T* A = gptr<const T>(d->a) + batch * d->ab + tm * BM * K;
         const TO_GLOBAL T* B = gptr<const T>(d->b) + batch * d->bb + tn * BN * K;
         T sa = T(1), sb = T(1);
@@ -1282,16 +1320,28 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
         for (int i = 0; i < NV; ++i) ra[i] = *reinterpret_cast<const TO_GLOBAL L*>(A + vrow[i] * K + vk[i]);
 #pragma unroll
         for (int i = 0; i < NVB; ++i) rb[i] = *reinterpret_cast<const TO_GLOBAL L*>(B + vrow[i] * K + vk[i]);
+        // rows keep k contiguous, so each 16-byte slab goes to LDS in one store; the
+        // operands' power-of-two scales are applied to the outputs instead (exact)
+        if (DB) {
+            __syncthreads();  // the previous tile's last slab has been read
+#pragma unroll
+            for (int i = 0; i < NV; ++i) *reinterpret_cast<L*>(&As[0][vrow[i]][vk[i]]) = ra[i];
+#pragma unroll
+            for (int i = 0; i < NVB; ++i) *reinterpret_cast<L*>(&Bs[0][vrow[i]][vk[i]]) = rb[i];
+            __syncthreads();
+        }
         for (int64_t k0 = 0; k0 < K; k0 += BK, ++step) {
-            // rows keep k contiguous, so each 16-byte slab goes to LDS in one store; the
-            // operands' power-of-two scales are applied to the outputs instead (exact)
-            __syncthreads();
+            const int cur = DB ? (int)(step & 1) : 0;
+            const bool more = k0 + BK < K;
+            if (!DB) {
+                __syncthreads();
 #pragma unroll
-            for (int i = 0; i < NV; ++i) *reinterpret_cast<L*>(&As[vrow[i]][vk[i]]) = ra[i];
+                for (int i = 0; i < NV; ++i) *reinterpret_cast<L*>(&As[0][vrow[i]][vk[i]]) = ra[i];
 #pragma unroll
-            for (int i = 0; i < NVB; ++i) *reinterpret_cast<L*>(&Bs[vrow[i]][vk[i]]) = rb[i];
-            __syncthreads();
-            if (k0 + BK < K) {
+                for (int i = 0; i < NVB; ++i) *reinterpret_cast<L*>(&Bs[0][vrow[i]][vk[i]]) = rb[i];
+                __syncthreads();
+            }
+            if (more) {
 #pragma unroll
                 for (int i = 0; i < NV; ++i)
                     ra[i] = *reinterpret_cast<const TO_GLOBAL L*>(A + vrow[i] * K + k0 + BK + vk[i]);
@@ -1304,16 +1354,26 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
                 const int kr = kk * 4 + (lane >> 4);
                 T a[4], b[NJ];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) a[i] = As[wm * 64 + i * 16 + (lane & 15)][kr];
+                for (int i = 0; i < 4; ++i) a[i] = As[cur][wm * 64 + i * 16 + (lane & 15)][kr];
 #pragma unroll
-                for (int j = 0; j < NJ; ++j) b[j] = Bs[wn * (BN / 2) + j * 16 + (lane & 15)][kr];
+                for (int j = 0; j < NJ; ++j) b[j] = Bs[cur][wn * (BN / 2) + j * 16 + (lane & 15)][kr];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int j = 0; j < NJ; ++j)
                         acc[i][j] = TR ? Mfma<T>::op(b[j], a[i], acc[i][j]) : Mfma<T>::op(a[i], b[j], acc[i][j]);
             }
-            if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 4, NJ>(acc, wide);
+            if (kWideAccum<T, WIDE> && ((step % FLUSH) == FLUSH - 1 || !more)) flush_wide<T, 4, NJ>(acc, wide);
+            if (DB) {
+                // the other buffer was last read before the previous barrier
+                if (more) {
+#pragma unroll
+                    for (int i = 0; i < NV; ++i) *reinterpret_cast<L*>(&As[cur ^ 1][vrow[i]][vk[i]]) = ra[i];
+#pragma unroll
+                    for (int i = 0; i < NVB; ++i) *reinterpret_cast<L*>(&Bs[cur ^ 1][vrow[i]][vk[i]]) = rb[i];
+                }
+                __syncthreads();
+            }
         }
 
         // Result element v of a lane: without TR, row (m) Mfma::row(lane, v) of column
@@ -1322,7 +1382,6 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
         // adjacent in C (rows with ldcm == 1, columns with TR and ldcn == 1) one 16-byte
         // store writes them.
         TO_GLOBAL T* C = gptr<T>(d->c) + batch * d->cb;
-        float mx = 0.0f;
         const T sab = sa * sb;
         const bool vec_out = sizeof(T) == 4 && (TR ? d->ldcn == 1 : d->ldcm == 1);
 #pragma unroll
@@ -1351,19 +1410,301 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
                 }
             }
         }
-        if (SCALED) {
-            mx = wave_max(mx);
-            if (lane == 0) note_max(d->cs, batch, mx);
-            if (tid == 0 && tm == 0 && tn == 0) note_exp(d->cs, batch, ea + eb);
-        }
+        if (SCALED && tid == 0 && tm == 0 && tn == 0) note_exp(d->cs, batch, ea + eb);
     }
+    if (SCALED) publish_max_on_change(d->cs, mx_batch, -1, mx, lane);
 }
 
 // 128x128 tiles with float accumulators, and 128x64 tiles for the wide (double) form,
 // both at two waves per SIMD
 template <typename T, bool SCALED, bool WIDE, bool TR>
 __global__ __launch_bounds__(kBlock, 2) void k_gemm128(const to_gemm_desc* __restrict__ descs, int ndesc) {
-    gemm128_body<T, SCALED, WIDE, WIDE ? 64 : 128, TR>(descs, ndesc);
+    gemm128_body<T, SCALED, WIDE, WIDE ? 64 : 128, TR, false>(descs, ndesc);
+}
+
+// 128x128 tiles at one block (one wave per SIMD) per CU: the register file of a SIMD
+// (accumulation registers included) holds the float and double accumulators of a 64x64
+// quarter, so the wide form keeps the 128-column tile; LDS double-buffered
+template <typename T, bool SCALED, bool WIDE, bool TR>
+__global__ __launch_bounds__(kBlock, 1) void k_gemm128_db(const to_gemm_desc* __restrict__ descs, int ndesc) {
+    gemm128_body<T, SCALED, WIDE, 128, TR, true>(descs, ndesc);
+}
+
+// ---------------------------------------------------------------------------------
+// Float products on the bf16 matrix cores (three-way split, six products).
+//
+// Every float operand x is split into three bf16 parts by round-to-nearest:
+// h = bf16(x), m = bf16(x - h), l = bf16(x - h - m); both residuals are exact in float and
+// x = h + m + l exactly (each step leaves at most 8 significant bits).  Then
+//   x * y = hh + (hm + mh) + (hl + mm + lh) + (ml + lm) + ll
+// and the six leading products are formed: the three dropped ones are below 2^-23 |x y|
+// and have random sign, so a product keeps float accuracy.  The bf16 products are exact
+// in the float accumulator of v_mfma_f32_16x16x32_bf16 (16 cycles for 16x16x32), so six
+// of them cost 6/16 of one float MFMA of the same depth (16x16x4 in 32 cycles, x8).
+// Inputs below 2^-110 of a tensor's largest entry lose bits in the bf16 subnormal range
+// (the float path loses them below 2^-126).
+//
+// 128x128 tiles, 4 waves with a 64x64 quarter each, K slabs of 32 floats fetched 16 bytes
+// at a time two slabs ahead; each float is split once per block when its slab is written
+// to LDS, as three bf16 planes [plane][row][k] whose row pitch (40 bf16 = 80 bytes) keeps
+// the 16-byte fragment reads conflict-free; LDS double-buffered, one block per CU.  MFMA
+// operand layout: lane i + 16g holds k = 8g..8g+7 of row (A) or column (B) i; the result
+// layout is that of the other 16x16 instructions.
+
+typedef __bf16 BF8 __attribute__((ext_vector_type(8)));
+typedef __bf16 BF2 __attribute__((ext_vector_type(2)));
+typedef float F2 __attribute__((ext_vector_type(2)));
+typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {  // a in the low half
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(F2{a, b}, BF2));
+}
+
+// the three bf16 planes of two floats, each pair packed in one word
+__device__ __forceinline__ void split3(float a, float b, uint32_t& h, uint32_t& m, uint32_t& l) {
+    h = pack_bf16(a, b);
+    a -= __uint_as_float(h << 16);
+    b -= __uint_as_float(h & 0xffff0000u);
+    m = pack_bf16(a, b);
+    a -= __uint_as_float(m << 16);
+    b -= __uint_as_float(m & 0xffff0000u);
+    l = pack_bf16(a, b);
+}
+
+__device__ __forceinline__ Mfma<float>::V mfma_bf16(U4 a, U4 b, Mfma<float>::V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(BF8, a), __builtin_bit_cast(BF8, b), c, 0,
+                                                  0, 0);
+}
+
+template <bool SCALED, bool WIDE, bool TR>
+__device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ descs, int ndesc) {
+    constexpr int BM = 128, BN = 128, BK = 32;
+    constexpr int PITCH = BK + 8;                    // bf16 per LDS row
+    constexpr int NV = BM * (BK / 4) / kBlock;       // 16-byte vectors per thread and operand (4)
+    constexpr int FLUSH = 4;                         // K steps per float partial sum (128 terms)
+    typedef Mfma<float>::V V;
+    typedef float L __attribute__((ext_vector_type(4)));
+    typedef uint32_t U2 __attribute__((ext_vector_type(2)));
+    __shared__ __align__(16) __bf16 As[2][3][BM][PITCH];
+    __shared__ __align__(16) __bf16 Bs[2][3][BN][PITCH];
+    const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t K = 1ll << d->klog;
+    const int64_t tiles_m = 1ll << (d->mlog - 7), tiles_n = 1ll << (d->nlog - 7);
+    const int64_t per_batch = tiles_m * tiles_n;
+    const int64_t ntiles = (int64_t)d->nbatch * per_batch;
+    int vrow[NV], vk[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int v = tid + kBlock * i;
+        vrow[i] = v / (BK / 4);
+        vk[i] = (v % (BK / 4)) * 4;
+    }
+    // fragment offset of this lane inside a plane: row (lane & 15), k = 8 * (lane >> 4)
+    const int frag = (lane & 15) * PITCH + 8 * (lane >> 4);
+    float mx = 0.0f;
+    int64_t mx_batch = -1;
+    for (int64_t t0 = blockIdx.x - d->block0; t0 < ntiles; t0 += d->nblocks) {
+        const int64_t t = xcd_tile(t0, ntiles);
+        const int64_t batch = t / per_batch;
+        const int64_t rem = t - batch * per_batch;
+        int64_t tm, tn;
+        grouped_tile(rem, tiles_m, tiles_n, tm, tn);
+        if (SCALED) publish_max_on_change(d->cs, mx_batch, batch, mx, lane);
+        const TO_GLOBAL float* A = gptr<const float>(d->a) + batch * d->ab + tm * BM * K;
+        const TO_GLOBAL float* B = gptr<const float>(d->b) + batch * d->bb + tn * BN * K;
+        float sa = 1.0f, sb = 1.0f;
+        int64_t ea = 0, eb = 0;
+        if (SCALED) {
+            const OpScale x = op_scale(d->as, d->asb, batch), y = op_scale(d->bs, d->bsb, batch);
+            sa = x.f; ea = x.e; sb = y.f; eb = y.e;
+        }
+        V acc[4][4];
+        double wide[4][4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    acc[i][j][v] = 0.0f;
+                    wide[i][j][v] = 0.0;
+                }
+        // NSET register sets of fetched slabs.  With two, slab s + 1 waits in one while
+        // slab s + 2 is fetched into the other, so a fetch has two K steps to arrive; the
+        // wide form (whose double accumulators take half the VGPRs) fetches one slab
+        // ahead.  Fetches past the end re-read the last slab (in bounds; what they stage
+        // is never read).
+        constexpr int NSET = WIDE ? 1 : 2;
+        static_assert(!WIDE || NSET == 1, "the wide form runs the one-slab-ahead loop");
+        L ra[NSET][NV], rb[NSET][NV];
+        auto fetch = [&](int set, int64_t k) {
+            k = k < K - BK ? k : K - BK;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                ra[set][i] = *reinterpret_cast<const TO_GLOBAL L*>(A + vrow[i] * K + k + vk[i]);
+                rb[set][i] = *reinterpret_cast<const TO_GLOBAL L*>(B + vrow[i] * K + k + vk[i]);
+            }
+        };
+        // split a fetched slab into the three planes of LDS buffer `buf`
+        auto stage = [&](int set, int buf) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                uint32_t h0, m0, l0, h1, m1, l1;
+                split3(ra[set][i][0], ra[set][i][1], h0, m0, l0);
+                split3(ra[set][i][2], ra[set][i][3], h1, m1, l1);
+                *reinterpret_cast<U2*>(&As[buf][0][vrow[i]][vk[i]]) = U2{h0, h1};
+                *reinterpret_cast<U2*>(&As[buf][1][vrow[i]][vk[i]]) = U2{m0, m1};
+                *reinterpret_cast<U2*>(&As[buf][2][vrow[i]][vk[i]]) = U2{l0, l1};
+                split3(rb[set][i][0], rb[set][i][1], h0, m0, l0);
+                split3(rb[set][i][2], rb[set][i][3], h1, m1, l1);
+                *reinterpret_cast<U2*>(&Bs[buf][0][vrow[i]][vk[i]]) = U2{h0, h1};
+                *reinterpret_cast<U2*>(&Bs[buf][1][vrow[i]][vk[i]]) = U2{m0, m1};
+                *reinterpret_cast<U2*>(&Bs[buf][2][vrow[i]][vk[i]]) = U2{l0, l1};
+            }
+        };
+        // The six products of LDS buffer `buf`, the large one last (a float accumulator
+        // that takes the small terms first rounds less): (l,h) (m,h) (m,m) (h,m) (h,l) (h,h).
+        // Consecutive products share one operand plane, so each step reads the fragments
+        // of one new plane (7 plane reads for 6 products) while the previous step's MFMAs
+        // run; scheduling barriers keep the steps in order, so two planes of fragments
+        // (plus the one being read) are live, not all six.
+        auto multiply = [&](int buf, bool fresh) {
+            U4 fa[4], fb[4], fn[4];
+            auto read_a = [&](U4 (&f)[4], int plane) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    f[i] = *reinterpret_cast<const U4*>(&As[buf][plane][wm * 64 + i * 16][0] + frag);
+            };
+            auto read_b = [&](U4 (&f)[4], int plane) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    f[j] = *reinterpret_cast<const U4*>(&Bs[buf][plane][wn * 64 + j * 16][0] + frag);
+            };
+            auto product = [&](const U4 (&a)[4], const U4 (&b)[4]) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = TR ? mfma_bf16(b[j], a[i], acc[i][j]) : mfma_bf16(a[i], b[j], acc[i][j]);
+            };
+            read_b(fb, 0);   // B h
+            read_a(fa, 2);   // A l
+            read_a(fn, 1);   // A m (next)
+            if (fresh) {     // the first product of a partial sum starts from zero
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = TR ? mfma_bf16(fb[j], fa[i], V{0.0f, 0.0f, 0.0f, 0.0f})
+                                       : mfma_bf16(fa[i], fb[j], V{0.0f, 0.0f, 0.0f, 0.0f});
+            } else {
+                product(fa, fb);                   // (l,h)
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            read_b(fa, 1);   // B m (next; A l is done)
+            product(fn, fb);                       // (m,h)
+            __builtin_amdgcn_sched_barrier(0);
+            read_a(fb, 0);   // A h (next; B h is done)
+            product(fn, fa);                       // (m,m)
+            __builtin_amdgcn_sched_barrier(0);
+            read_b(fn, 2);   // B l (next; A m is done)
+            product(fb, fa);                       // (h,m)
+            __builtin_amdgcn_sched_barrier(0);
+            read_b(fa, 0);   // B h again (next; B m is done)
+            product(fb, fn);                       // (h,l)
+            __builtin_amdgcn_sched_barrier(0);
+            product(fb, fa);                       // (h,h)
+        };
+        constexpr int S1 = NSET - 1;  // the set of odd slabs
+        fetch(0, 0);
+        if (NSET == 2) fetch(S1, BK);
+        __syncthreads();  // the previous tile's last slab has been read
+        stage(0, 0);
+        __syncthreads();
+        if (NSET == 1) {
+            // K step s: fetch slab s + 1, multiply buffer s & 1, stage slab s + 1 into the
+            // other buffer; one barrier per step.  Unrolled by the partial-sum length (the
+            // host requires K >= FLUSH * BK): its first product starts from zero and its
+            // last step adds the float partial sums into the double accumulators.
+            for (int64_t k0 = 0; k0 < K; k0 += FLUSH * BK) {
+#pragma unroll
+                for (int u = 0; u < FLUSH; ++u) {
+                    const int64_t kk = k0 + u * BK;
+                    const bool more = kk + BK < K;
+                    if (more) fetch(0, kk + BK);
+                    multiply(u & 1, u == 0);
+                    if (u == FLUSH - 1) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                                for (int v = 0; v < 4; ++v) wide[i][j][v] += (double)acc[i][j][v];
+                    }
+                    if (more) stage(0, (u & 1) ^ 1);
+                    __syncthreads();
+                }
+            }
+        } else {
+            // K step s multiplies buffer s & 1 while slab s + 2 is fetched into set s & 1
+            // and slab s + 1 (set and buffer (s + 1) & 1) is staged; one barrier per step
+            // (scheduling barriers keep the phases apart, which bounds the registers live
+            // at once)
+            for (int64_t k0 = 0; k0 < K; k0 += 2 * BK) {
+                fetch(0, k0 + 2 * BK);
+                multiply(0, false);
+                __builtin_amdgcn_sched_barrier(0);
+                stage(S1, 1);
+                __syncthreads();
+                if (k0 + BK >= K) break;  // a single slab (K == BK)
+                fetch(S1, k0 + 3 * BK);
+                multiply(1, false);
+                __builtin_amdgcn_sched_barrier(0);
+                stage(0, 0);
+                __syncthreads();
+            }
+        }
+
+        TO_GLOBAL float* C = gptr<float>(d->c) + batch * d->cb;
+        const float sab = sa * sb;
+        const bool vec_out = TR ? d->ldcn == 1 : d->ldcm == 1;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float out[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    out[v] = (WIDE ? (float)wide[i][j][v] : acc[i][j][v]) * sab;
+                    if (SCALED) mx = fmaxf(mx, fabsf(out[v]));
+                }
+                const int64_t mb = tm * BM + wm * 64 + i * 16, nb = tn * BN + wn * 64 + j * 16;
+                if (vec_out) {
+                    const int64_t m = TR ? mb + (lane & 15) : mb + Mfma<float>::row(lane, 0);
+                    const int64_t n = TR ? nb + Mfma<float>::row(lane, 0) : nb + (lane & 15);
+                    *reinterpret_cast<TO_GLOBAL L*>(C + m * d->ldcm + n * d->ldcn) = L{out[0], out[1], out[2], out[3]};
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int64_t m = TR ? mb + (lane & 15) : mb + Mfma<float>::row(lane, v);
+                        const int64_t n = TR ? nb + Mfma<float>::row(lane, v) : nb + (lane & 15);
+                        C[m * d->ldcm + n * d->ldcn] = out[v];
+                    }
+                }
+            }
+        }
+        if (SCALED && tid == 0 && tm == 0 && tn == 0) note_exp(d->cs, batch, ea + eb);
+    }
+    if (SCALED) publish_max_on_change(d->cs, mx_batch, -1, mx, lane);
+}
+
+// one block per CU (the double accumulators need the register file of one wave per SIMD),
+// LDS double-buffered
+template <bool SCALED, bool WIDE, bool TR>
+__global__ __launch_bounds__(kBlock, 1) void k_gemm_x6(const to_gemm_desc* __restrict__ descs, int ndesc) {
+    gemm_x6_body<SCALED, WIDE, TR>(descs, ndesc);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1463,6 +1804,43 @@ int launch_one(const to_launch& L, hipStream_t st) {
             else if (L.variant == 7)
                 hipLaunchKernelGGL((k_gemm128<T, SCALED, true, true>), grid, block, 0, st,
                                    reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+            else if (L.variant == 9)
+                hipLaunchKernelGGL((k_gemm128_db<T, SCALED, false, false>), grid, block, 0, st,
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+            else if (L.variant == 11)
+                hipLaunchKernelGGL((k_gemm128_db<T, SCALED, true, false>), grid, block, 0, st,
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+            else if (L.variant == 13)
+                hipLaunchKernelGGL((k_gemm128_db<T, SCALED, false, true>), grid, block, 0, st,
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+            else if (L.variant == 15)
+                hipLaunchKernelGGL((k_gemm128_db<T, SCALED, true, true>), grid, block, 0, st,
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+            else if (L.variant & 16) {
+                if constexpr (sizeof(T) == 4) {
+                    const to_gemm_desc* gd = reinterpret_cast<const to_gemm_desc*>(L.descs);
+                    switch (L.variant) {
+                        case 25:
+                            hipLaunchKernelGGL((k_gemm_x6<SCALED, false, false>), grid, block, 0, st, gd, L.count);
+                            break;
+                        case 27:
+                            hipLaunchKernelGGL((k_gemm_x6<SCALED, true, false>), grid, block, 0, st, gd, L.count);
+                            break;
+                        case 29:
+                            hipLaunchKernelGGL((k_gemm_x6<SCALED, false, true>), grid, block, 0, st, gd, L.count);
+                            break;
+                        case 31:
+                            hipLaunchKernelGGL((k_gemm_x6<SCALED, true, true>), grid, block, 0, st, gd, L.count);
+                            break;
+                        default:  // the split product runs at one block per CU (bit 3)
+                            set_err("gemm: unknown bf16 split-product variant");
+                            return TO_ERR_INVALID;
+                    }
+                } else {
+                    set_err("gemm: the bf16 split product takes float operands");
+                    return TO_ERR_INVALID;
+                }
+            }
             else if (L.variant == 0)
                 hipLaunchKernelGGL((k_gemm<T, SCALED, false>), grid, block, 0, st,
                                    reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);

@@ -223,7 +223,15 @@ STREAM_VEC = os.environ.get("TENSORORDER_STREAM_VEC", "1") != "0"
 WIDE_ACCUM_MIN_SHARED = int(os.environ.get("TENSORORDER_WIDE_ACCUM_MIN_SHARED", "11"))
 GEMM_MIN_FREE = 6
 GEMM_MIN_SHARED = 4
+GEMM128_MIN_SHARED = {(8, False): 4, (4, False): 5, (4, True): 5}
 MAX_GEMM_BLOCKS = 8192
+# by rule, wide float products stay on 128x64 tiles at two blocks per CU (measured faster
+# on MI355X than 128x128 at one block per CU); TENSORORDER_GEMM_WIDE_DB=1 picks the latter
+# (the autotuner chooses per shape either way)
+GEMM_WIDE_DB = os.environ.get("TENSORORDER_GEMM_WIDE_DB", "0") != "0"
+# float products with 128x128 tiles on the bf16 matrix cores, each operand split into three
+# bf16 parts (TENSORORDER_GEMM_X6=0 keeps them on the float MFMA)
+GEMM_X6 = os.environ.get("TENSORORDER_GEMM_X6", "1") != "0"
 
 
 def _canonical(free_pos, shared_pos):
@@ -250,19 +258,30 @@ def choose_variant(g, nb, esize):
     K = 1 << s
     if USE_GEMM and fx >= GEMM_MIN_FREE and fy >= GEMM_MIN_FREE and s >= GEMM_MIN_SHARED:
         wide = esize == 4 and s >= WIDE_ACCUM_MIN_SHARED
-        # 128-row tiles: 128 columns with float accumulators, 64 with double (wide)
-        tn = 6 if wide else 7
-        big = GEMM_TILES == 128 and fx >= 7 and fy >= tn and s >= (4 if esize == 8 else 5)
+        # 128-row tiles: 128 columns, except the wide (double-accumulating) form at two
+        # blocks per CU, which has registers for 64; at one block per CU (bit 3, LDS
+        # double-buffered) the wide form keeps 128 columns
+        # K slab of the 128-row kernels: 16 doubles, 32 floats
+        big = GEMM_TILES == 128 and fx >= 7 and fy >= (6 if wide else 7) and s >= GEMM128_MIN_SHARED[(esize, wide)]
+        # float products on the bf16 matrix cores (three-way split, bit 4): 128x128 tiles
+        # at one block per CU (bit 3)
+        x6 = big and esize == 4 and fy >= 7 and GEMM_X6 and (s >= 7 or not wide)
+        db = big and fy >= 7 and ((wide and GEMM_WIDE_DB) or x6)
         cap = MAX_GEMM_BLOCKS
         tuned = TUNED.get(gemm_key(esize, fx, fy, s, nb))
-        if tuned is not None and bool(tuned["variant"] & 2) == wide and (big or not tuned["variant"] & 1):
+        if tuned is not None and bool(tuned["variant"] & 2) == wide and (big or not tuned["variant"] & 1) \
+                and (GEMM_X6 or not tuned["variant"] & 16) and (s >= 7 or not (wide and tuned["variant"] & 16)):
             big = bool(tuned["variant"] & 1)  # the tile shape; bit 2 follows the layout below
+            db = bool(tuned["variant"] & 8)
+            x6 = bool(tuned["variant"] & 16)
             cap = tuned.get("block_cap") or MAX_GEMM_BLOCKS
+        tn = 6 if (wide and not db) else 7
         tiles = nb << ((fx - 7 + fy - tn) if big else (fx - 6 + fy - 6))
         # float outputs whose columns are contiguous (shy == 0) are written 16 bytes at a
         # time from a transposed product
         tr = big and esize == 4 and g["shy"] == 0
-        return dict(kind="gemm", tile=(1 if big else 0) | (2 if wide else 0) | (4 if tr else 0),
+        return dict(kind="gemm", tile=(1 if big else 0) | (2 if wide else 0) | (4 if tr else 0) | (8 if db else 0)
+                    | (16 if x6 else 0),
                     nblocks=int(min(tiles, cap)),
                     perm_x=not _canonical(g["xf"], g["xs"]), perm_y=not _canonical(g["yf"], g["ys"]))
     yc = min(16, 1 << fy)

@@ -193,9 +193,12 @@ def test_gemm_descriptors_fit_their_tiles(et):
         for variant, gd in _gemm_descs(prog):
             seen += 1
             if variant & 1:
-                assert gd.mlog >= 7 and gd.nlog >= (6 if variant & 2 else 7)
+                assert gd.mlog >= 7 and gd.nlog >= (6 if (variant & 2 and not variant & 8) else 7)
                 assert gd.klog >= (5 if f32 else 4)  # K slab of 32 floats / 16 doubles
             else:
                 assert gd.mlog >= 6 and gd.nlog >= 6 and gd.klog >= 4
             assert not (variant & 2) or f32
+            if variant & 16:  # bf16 split products: float, 128x128 tiles, one block per CU
+                assert f32 and variant & 1 and variant & 8 and gd.nlog >= 7
+                assert gd.klog >= (7 if variant & 2 else 5)  # wide: whole partial sums of 4 slabs
     assert seen > 0

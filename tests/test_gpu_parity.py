@@ -230,3 +230,97 @@ def test_batch_fixture_instances(hip, hip32):
             assert [r.value for r in res] == pytest.approx(wants, rel=tol)
         finally:
             exe.close()
+
+
+def _gemm_variants(esize, mlog, nlog, klog):
+    # K slabs: 16 doubles, 32 floats
+    out = [0] + ([2] if esize == 4 else [])
+    if mlog >= 7 and klog >= (5 if esize == 4 else 4) and nlog >= 7:
+        out += [1, 5, 9, 13]
+    if esize == 4 and mlog >= 7 and klog >= 5:
+        if nlog >= 6:
+            out += [3, 7]
+        if nlog >= 7:
+            out += [11, 15]
+    if esize == 4 and mlog >= 7 and nlog >= 7 and klog >= 5:
+        out += [25, 29]  # bf16 three-way split products
+        if klog >= 7:
+            out += [27, 31]  # wide: K in whole partial sums of 4 slabs
+    return out
+
+
+@pytest.mark.parametrize("et", ["f64", "f32s"])
+def test_gemm_every_tile_variant(hip, hip32, et):
+    # every MFMA product kernel (64x64, 128x128, 128x64 wide, 128x128 one-block-per-CU
+    # double-buffered, bf16 three-way split; plain and transposed) on the same operands,
+    # both output layouts, with fewer blocks than tiles, against the float64 product
+    import ctypes
+
+    import torch
+
+    from tensororder_amd import _lib
+
+    api = hip if et == "f64" else hip32
+    esize = api.get_entry_size()
+    dt = np.float64 if esize == 8 else np.float32
+    rng = np.random.default_rng(11)
+    shapes = [(7, 7, 4), (8, 7, 6)] if esize == 8 else [(7, 7, 5), (7, 7, 6), (8, 8, 11), (8, 6, 12), (7, 8, 13)]
+    dev = torch.device("cuda", 0)
+    for mlog, nlog, klog in shapes:
+        nb = 2
+        M, N, K = 1 << mlog, 1 << nlog, 1 << klog
+        a = rng.uniform(0.5, 2.0, (nb, M, K)).astype(dt)
+        b = rng.uniform(0.5, 2.0, (nb, N, K)).astype(dt)
+        want = np.einsum("bmk,bnk->bmn", a.astype(np.float64), b.astype(np.float64))
+        ta = torch.from_numpy(a).to(dev)
+        tb = torch.from_numpy(b).to(dev)
+        err = {}
+        for variant in _gemm_variants(esize, mlog, nlog, klog):
+            for cols_contiguous in (True, False):
+                _say("case gemm", et, mlog, nlog, klog, "variant", variant, "ldcn==1" if cols_contiguous else "ldcm==1")
+                tc = torch.full((nb, M * N), float("nan"), dtype=ta.dtype, device=dev)
+                gd = _lib.GemmDesc()
+                gd.a, gd.b, gd.c = ta.data_ptr(), tb.data_ptr(), tc.data_ptr()
+                gd.ab, gd.bb, gd.cb = M * K, N * K, M * N
+                gd.ldcm, gd.ldcn = (N, 1) if cols_contiguous else (1, M)
+                recs = None
+                if esize == 4:
+                    rs = np.zeros((3 * nb, 2), dtype=np.int64)  # exp, maxbits | pad
+                    for i in range(nb):
+                        rs[i, 1] = int(np.float32(np.abs(a[i]).max()).view(np.uint32))
+                        rs[nb + i, 1] = int(np.float32(np.abs(b[i]).max()).view(np.uint32))
+                    recs = torch.from_numpy(rs).to(dev)
+                    rsz = ctypes.sizeof(_lib.ScaleRec)
+                    gd.as_, gd.bs, gd.cs = recs.data_ptr(), recs.data_ptr() + nb * rsz, recs.data_ptr() + 2 * nb * rsz
+                    gd.asb = gd.bsb = 1
+                gd.mlog, gd.nlog, gd.klog = mlog, nlog, klog
+                tiles = nb * (M // (64 if variant in (0, 2) else 128)) * (N // (64 if variant in (0, 2, 3, 7) else 128))
+                gd.nbatch, gd.block0, gd.nblocks = nb, 0, max(1, min(3, tiles))
+                blob = torch.frombuffer(bytearray(bytes(gd)), dtype=torch.uint8).to(dev)
+                torch.cuda.synchronize()
+                L = _lib.Launch(op=_lib.TO_OP_GEMM, variant=variant, count=1, grid=gd.nblocks,
+                                descs=blob.data_ptr())
+                arr = _lib.launch_array([L])
+                _lib.check(api._lib.to_run(ctypes.addressof(arr), 1, api._entry_type,
+                                           ctypes.c_void_p(api._stream.cuda_stream)), "to_run")
+                api._stream.synchronize()
+                c = tc.cpu().numpy().reshape(nb, M, N) if cols_contiguous else \
+                    tc.cpu().numpy().reshape(nb, N, M).transpose(0, 2, 1)
+                if esize == 8:
+                    np.testing.assert_allclose(c, want, rtol=1e-12)
+                else:
+                    out = recs.cpu().numpy()[2 * nb:]
+                    for i in range(nb):
+                        assert int(out[i, 1]) == int(np.float32(np.abs(c[i]).max()).view(np.uint32))
+                        got = c[i].astype(np.float64) * 2.0 ** int(out[i, 0])
+                        # float operands, float MFMA sums (in double every 4 K steps when wide)
+                        np.testing.assert_allclose(got, want[i], rtol=2e-6 if variant & 2 else 1e-5)
+                        err[variant] = max(err.get(variant, 0.0), float(np.max(np.abs(got - want[i]) / want[i])))
+        if esize == 4:
+            # the bf16 split products keep float accuracy: no worse than twice the float
+            # MFMA kernel's largest error with the same accumulation
+            _say("max rel error by variant", {v: "%.2e" % e for v, e in sorted(err.items())})
+            for v, e in err.items():
+                base = 3 if v & 2 else 1
+                if v & 16 and base in err:
+                    assert e <= 2 * err[base] + 1e-7, (v, e, err[base])
