@@ -1494,12 +1494,15 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
     const int64_t tiles_m = 1ll << (d->mlog - 7), tiles_n = 1ll << (d->nlog - 7);
     const int64_t per_batch = tiles_m * tiles_n;
     const int64_t ntiles = (int64_t)d->nbatch * per_batch;
+    // Vector i of a thread: 16 bytes (4 k) of one row of a slab; the 8 lanes of a row take
+    // its 8 vectors (128 contiguous bytes), and lanes 8q..8q+7 of a wave take rows
+    // (q >> 1) + 4 (q & 1) of its 8, so that the 16 lanes of one 8-byte LDS store cycle
+    // write rows r and r + 4, which lie in opposite halves of the banks (pitch 80 bytes).
     int vrow[NV], vk[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        const int v = tid + kBlock * i;
-        vrow[i] = v / (BK / 4);
-        vk[i] = (v % (BK / 4)) * 4;
+        vrow[i] = 32 * i + 8 * wave + ((lane >> 4) | (((lane >> 3) & 1) << 2));
+        vk[i] = (lane & 7) * 4;
     }
     // fragment offset of this lane inside a plane: row (lane & 15), k = 8 * (lane >> 4)
     const int frag = (lane & 15) * PITCH + 8 * (lane >> 4);
@@ -1531,46 +1534,45 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
                     acc[i][j][v] = 0.0f;
                     wide[i][j][v] = 0.0;
                 }
-        // NSET register sets of fetched slabs.  With two, slab s + 1 waits in one while
-        // slab s + 2 is fetched into the other, so a fetch has two K steps to arrive; the
-        // wide form (whose double accumulators take half the VGPRs) fetches one slab
-        // ahead.  Fetches past the end re-read the last slab (in bounds; what they stage
-        // is never read).
-        constexpr int NSET = WIDE ? 1 : 2;
-        static_assert(!WIDE || NSET == 1, "the wide form runs the one-slab-ahead loop");
-        L ra[NSET][NV], rb[NSET][NV];
-        auto fetch = [&](int set, int64_t k) {
+        // Software pipeline over K slabs of 32: while K step s multiplies LDS buffer
+        // s & 1, each of the thread's 8 fetched vectors of slab s + 1 (4 of A, 4 of B) is
+        // split into the other buffer between two products, and its register is refilled
+        // at once from slab s + 2.  A vector is thus split at the same point of every step,
+        // one step after its fetch, and the splitting (VALU) runs beside the MFMAs.
+        // Fetches past the end re-read the last slab (in bounds; what they stage is never
+        // read).  One barrier per step.
+        L rv[2 * NV];
+        auto fetch_vec = [&](int v, int64_t k) {
             k = k < K - BK ? k : K - BK;
-#pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                ra[set][i] = *reinterpret_cast<const TO_GLOBAL L*>(A + vrow[i] * K + k + vk[i]);
-                rb[set][i] = *reinterpret_cast<const TO_GLOBAL L*>(B + vrow[i] * K + k + vk[i]);
-            }
+            const TO_GLOBAL float* src = v < NV ? A : B;
+            const int i = v & (NV - 1);
+            rv[v] = *reinterpret_cast<const TO_GLOBAL L*>(src + vrow[i] * K + k + vk[i]);
         };
-        // split a fetched slab into the three planes of LDS buffer `buf`
-        auto stage = [&](int set, int buf) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                uint32_t h0, m0, l0, h1, m1, l1;
-                split3(ra[set][i][0], ra[set][i][1], h0, m0, l0);
-                split3(ra[set][i][2], ra[set][i][3], h1, m1, l1);
-                *reinterpret_cast<U2*>(&As[buf][0][vrow[i]][vk[i]]) = U2{h0, h1};
-                *reinterpret_cast<U2*>(&As[buf][1][vrow[i]][vk[i]]) = U2{m0, m1};
-                *reinterpret_cast<U2*>(&As[buf][2][vrow[i]][vk[i]]) = U2{l0, l1};
-                split3(rb[set][i][0], rb[set][i][1], h0, m0, l0);
-                split3(rb[set][i][2], rb[set][i][3], h1, m1, l1);
-                *reinterpret_cast<U2*>(&Bs[buf][0][vrow[i]][vk[i]]) = U2{h0, h1};
-                *reinterpret_cast<U2*>(&Bs[buf][1][vrow[i]][vk[i]]) = U2{m0, m1};
-                *reinterpret_cast<U2*>(&Bs[buf][2][vrow[i]][vk[i]]) = U2{l0, l1};
-            }
+        auto stage_vec = [&](int v, int buf) {
+            const int i = v & (NV - 1);
+            uint32_t h0, m0, l0, h1, m1, l1;
+            split3(rv[v][0], rv[v][1], h0, m0, l0);
+            split3(rv[v][2], rv[v][3], h1, m1, l1);
+            __bf16 (*P)[BM][PITCH] = v < NV ? As[buf] : Bs[buf];
+            *reinterpret_cast<U2*>(&P[0][vrow[i]][vk[i]]) = U2{h0, h1};
+            *reinterpret_cast<U2*>(&P[1][vrow[i]][vk[i]]) = U2{m0, m1};
+            *reinterpret_cast<U2*>(&P[2][vrow[i]][vk[i]]) = U2{l0, l1};
+        };
+        // stage two vectors of slab s + 1 into `nbuf` and fetch them from slab s + 2 (at k2)
+        auto pipe = [&](int v, int nbuf, int64_t k2) {
+            stage_vec(v, nbuf);
+            stage_vec(v + 1, nbuf);
+            fetch_vec(v, k2);
+            fetch_vec(v + 1, k2);
         };
         // The six products of LDS buffer `buf`, the large one last (a float accumulator
         // that takes the small terms first rounds less): (l,h) (m,h) (m,m) (h,m) (h,l) (h,h).
         // Consecutive products share one operand plane, so each step reads the fragments
-        // of one new plane (7 plane reads for 6 products) while the previous step's MFMAs
-        // run; scheduling barriers keep the steps in order, so two planes of fragments
-        // (plus the one being read) are live, not all six.
-        auto multiply = [&](int buf, bool fresh) {
+        // of one new plane (7 plane reads for 6 products) while the previous product's
+        // MFMAs run; scheduling barriers keep the products in order, so two planes of
+        // fragments (plus the one being read) are live, not all six.
+        auto step = [&](int buf, bool fresh, int64_t k2) {
+            const int nbuf = buf ^ 1;
             U4 fa[4], fb[4], fn[4];
             auto read_a = [&](U4 (&f)[4], int plane) {
 #pragma unroll
@@ -1592,6 +1594,17 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
             read_b(fb, 0);   // B h
             read_a(fa, 2);   // A l
             read_a(fn, 1);   // A m (next)
+            // Interleave the splitting of two vectors (about 44 VALU) with the 16 MFMAs of
+            // one product: while an MFMA runs (16 cycles), the wave issues the next VALU
+            // instructions.
+            auto interleave = [&]() {
+#pragma unroll
+                for (int q = 0; q < 11; ++q) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // four VALU
+                }
+                __builtin_amdgcn_sched_group_barrier(0x008, 5, 0);
+            };
             if (fresh) {     // the first product of a partial sum starts from zero
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
@@ -1602,67 +1615,56 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
             } else {
                 product(fa, fb);                   // (l,h)
             }
+            pipe(0, nbuf, k2);
+            interleave();
             __builtin_amdgcn_sched_barrier(0);
             read_b(fa, 1);   // B m (next; A l is done)
             product(fn, fb);                       // (m,h)
+            pipe(2, nbuf, k2);
+            interleave();
             __builtin_amdgcn_sched_barrier(0);
             read_a(fb, 0);   // A h (next; B h is done)
             product(fn, fa);                       // (m,m)
+            pipe(4, nbuf, k2);
+            interleave();
             __builtin_amdgcn_sched_barrier(0);
             read_b(fn, 2);   // B l (next; A m is done)
             product(fb, fa);                       // (h,m)
+            pipe(6, nbuf, k2);
+            interleave();
             __builtin_amdgcn_sched_barrier(0);
             read_b(fa, 0);   // B h again (next; B m is done)
             product(fb, fn);                       // (h,l)
             __builtin_amdgcn_sched_barrier(0);
             product(fb, fa);                       // (h,h)
         };
-        constexpr int S1 = NSET - 1;  // the set of odd slabs
-        fetch(0, 0);
-        if (NSET == 2) fetch(S1, BK);
+#pragma unroll
+        for (int v = 0; v < 2 * NV; ++v) fetch_vec(v, 0);
         __syncthreads();  // the previous tile's last slab has been read
-        stage(0, 0);
+#pragma unroll
+        for (int v = 0; v < 2 * NV; ++v) {
+            stage_vec(v, 0);
+            fetch_vec(v, BK);
+        }
         __syncthreads();
-        if (NSET == 1) {
-            // K step s: fetch slab s + 1, multiply buffer s & 1, stage slab s + 1 into the
-            // other buffer; one barrier per step.  Unrolled by the partial-sum length (the
-            // host requires K >= FLUSH * BK): its first product starts from zero and its
-            // last step adds the float partial sums into the double accumulators.
-            for (int64_t k0 = 0; k0 < K; k0 += FLUSH * BK) {
+        // unrolled by the partial-sum length when wide (the host requires K >= FLUSH * BK):
+        // its first product starts from zero and its last step adds the float partial sums
+        // into the double accumulators; by the buffer parity otherwise
+        constexpr int UNR = WIDE ? FLUSH : 2;
+        for (int64_t k0 = 0; k0 < K; k0 += UNR * BK) {
 #pragma unroll
-                for (int u = 0; u < FLUSH; ++u) {
-                    const int64_t kk = k0 + u * BK;
-                    const bool more = kk + BK < K;
-                    if (more) fetch(0, kk + BK);
-                    multiply(u & 1, u == 0);
-                    if (u == FLUSH - 1) {
+            for (int u = 0; u < UNR; ++u) {
+                const int64_t kk = k0 + u * BK;
+                if (!WIDE && u > 0 && kk >= K) break;  // K == BK
+                step(u & 1, WIDE && u == 0, kk + 2 * BK);
+                if (WIDE && u == UNR - 1) {
 #pragma unroll
-                        for (int i = 0; i < 4; ++i)
+                    for (int i = 0; i < 4; ++i)
 #pragma unroll
-                            for (int j = 0; j < 4; ++j)
+                        for (int j = 0; j < 4; ++j)
 #pragma unroll
-                                for (int v = 0; v < 4; ++v) wide[i][j][v] += (double)acc[i][j][v];
-                    }
-                    if (more) stage(0, (u & 1) ^ 1);
-                    __syncthreads();
+                            for (int v = 0; v < 4; ++v) wide[i][j][v] += (double)acc[i][j][v];
                 }
-            }
-        } else {
-            // K step s multiplies buffer s & 1 while slab s + 2 is fetched into set s & 1
-            // and slab s + 1 (set and buffer (s + 1) & 1) is staged; one barrier per step
-            // (scheduling barriers keep the phases apart, which bounds the registers live
-            // at once)
-            for (int64_t k0 = 0; k0 < K; k0 += 2 * BK) {
-                fetch(0, k0 + 2 * BK);
-                multiply(0, false);
-                __builtin_amdgcn_sched_barrier(0);
-                stage(S1, 1);
-                __syncthreads();
-                if (k0 + BK >= K) break;  // a single slab (K == BK)
-                fetch(S1, k0 + 3 * BK);
-                multiply(1, false);
-                __builtin_amdgcn_sched_barrier(0);
-                stage(0, 0);
                 __syncthreads();
             }
         }
