@@ -50,8 +50,11 @@ extern "C" {
 #define TO_OP_JOIN_STREAM 2  /* descs: to_join_desc[count]; variant = y values per thread (YC),
                                 or TO_STREAM_VEC | YC for the 16-byte form: fx >= 10 (float) /
                                 9 (double), the lowest 2 / 1 bits of X free, shy == 0,
-                                shf == fy, YC == 2^fy, LDS 8*2^s (16-aligned) + YC*2^s entries */
+                                shf == fy, YC == 2^fy, LDS 8*2^s (16-aligned) + YC*2^s entries;
+                                or TO_STREAM_ROWS | s for the row form: shy == 0, shf == fy,
+                                5 <= fy <= 10 (float) / 9 (double), fx >= 6, s <= 4, no LDS */
 #define TO_STREAM_VEC 0x100
+#define TO_STREAM_ROWS 0x200
 #define TO_OP_JOIN_SPLITK 3  /* descs: to_join_desc[count]; partial sums (double) to d.partial */
 #define TO_OP_JOIN_FINISH 4  /* descs: to_join_desc[count]; sums d.partial into C; variant 1:
                                 one block per output (for nsplit >= 32) */

@@ -124,9 +124,15 @@ __device__ __forceinline__ OpScale op_scale(uint64_t recs, int64_t stride, int64
     return OpScale{ldexpf(1.0f, -e), r->exp + e};
 }
 
+// Raise the running max |entry| of batch item b to m (non-negative floats compare as their
+// bits).  The atomic is skipped when the stored max (read from L2, so never above the true
+// current value) is already at least m: all the waves of a launch publish into one record,
+// and atomics on one address are serialised, so most of them would otherwise queue.
 __device__ __forceinline__ void note_max(uint64_t recs, int64_t b, float m) {
     TO_GLOBAL to_scale_rec* r = gptr<to_scale_rec>(recs) + b;
-    atomicMax((uint32_t*)&r->maxbits, __float_as_uint(m));
+    uint32_t* p = (uint32_t*)&r->maxbits;
+    const uint32_t bits = __float_as_uint(m);
+    if (bits > __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(p, bits);
 }
 
 __device__ __forceinline__ void note_exp(uint64_t recs, int64_t b, int64_t e) {
@@ -751,6 +757,95 @@ __global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* 
         } else {
             xhi = ((xhi | ~hmask) + 1) & hmask;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// streaming join, row form: outputs C[f][y] with the 2^fy values of y contiguous
+// (shy == 0, shf == fy, fy >= 5) and few shared bits (s = S <= 4) -- an outer product of
+// long rows, bound by the output writes.  Each thread owns VY = 16 bytes of consecutive y
+// of a row and keeps their Y[y][k] (VY * 2^S values) in registers for its batch item;
+// the threads of one row read the same X[f][k] (one load per wave), form VY outputs and
+// write them as one 16-byte store, so a wave writes whole rows.  A block walks a
+// contiguous range of tiles of 64 rows of one batch item.
+
+template <typename T, bool SCALED, int S>
+__global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __restrict__ descs, int ndesc) {
+    constexpr int K = 1 << S;
+    constexpr int VY = 16 / sizeof(T);
+    constexpr int ROWS = 64;  // rows per tile
+    typedef typename Vec16<T>::type V;
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int tid = threadIdx.x;
+    const int fx = d->fx, fy = d->fy;
+    const int tpr = (1 << fy) / VY;          // threads per row (the host keeps it <= kBlock)
+    const int rb = kBlock / tpr;             // rows per pass
+    const int j = tid % tpr, r = tid / tpr;  // y vector and row of this thread
+    const TO_GLOBAL T* __restrict__ X = gptr<const T>(d->x);
+    const TO_GLOBAL T* __restrict__ Y = gptr<const T>(d->y);
+    TO_GLOBAL T* __restrict__ C = gptr<T>(d->c);
+    int64_t kx[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) kx[k] = uniform64(pdep(k, d->xs_pos, S));
+    const int local = blockIdx.x - d->block0;
+    const int tb = fx - 6;                   // tile bits of f
+    const int64_t nft = 1ll << tb;
+    const int64_t ntiles = (int64_t)d->nbatch << tb;
+    const int64_t t0 = ntiles * local / d->nblocks, t1 = ntiles * (local + 1) / d->nblocks;
+    const int64_t fmask = pdep(~0ull, d->xf_pos, fx);
+    const int64_t xstep = pdep((uint64_t)rb, d->xf_pos, fx);  // rb rows on, in X's bit layout
+    T yr[K][VY];
+    int64_t staged = -1;
+    float sxy = 1.0f;
+    int64_t exy = 0;
+    float mrun = 0.0f;
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t batch = t >> tb, ft = t & (nft - 1);
+        if (batch != staged) {
+            if (SCALED) {
+                if (staged >= 0) {
+                    const float w = wave_max(mrun);
+                    if ((tid & 63) == 0) note_max(d->cs, staged, w);
+                    mrun = 0.0f;
+                }
+                const OpScale a = op_scale(d->xs, d->xsb, batch), c = op_scale(d->ys, d->ysb, batch);
+                sxy = a.f * c.f;
+                exy = a.e + c.e;
+            }
+            const TO_GLOBAL T* Yb = Y + batch * d->yb;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int64_t ky = pdep(k, d->ys_pos, S);
+#pragma unroll
+                for (int e = 0; e < VY; ++e) yr[k][e] = Yb[pdep(j * VY + e, d->yf_pos, fy) + ky];
+            }
+            staged = batch;
+        }
+        if (SCALED && ft == 0 && tid == 0) note_exp(d->cs, batch, exy);
+        const TO_GLOBAL T* Xb = X + batch * d->xb;
+        TO_GLOBAL T* Cb = C + batch * d->cb;
+        const int64_t f0 = ft * ROWS + r;
+        int64_t xo = pdep((uint64_t)f0, d->xf_pos, fx);
+        for (int64_t f = f0; f < ft * ROWS + ROWS; f += rb) {
+            T xv[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) xv[k] = Xb[xo + kx[k]];
+            V o;
+#pragma unroll
+            for (int e = 0; e < VY; ++e) {
+                T a = T(0);
+#pragma unroll
+                for (int k = 0; k < K; ++k) a = fma(xv[k], yr[k][e], a);
+                o[e] = SCALED ? a * T(sxy) : a;
+                if (SCALED) mrun = fmaxf(mrun, (float)absval(o[e]));
+            }
+            *reinterpret_cast<TO_GLOBAL V*>(Cb + (f << fy) + j * VY) = o;
+            xo = ((xo | ~fmask) + xstep) & fmask;
+        }
+    }
+    if (SCALED && staged >= 0) {
+        const float w = wave_max(mrun);
+        if ((tid & 63) == 0) note_max(d->cs, staged, w);
     }
 }
 
@@ -1744,6 +1839,16 @@ int launch_one(const to_launch& L, hipStream_t st) {
                 TO_STREAM_CASE(8)
                 TO_STREAM_CASE(16)
 #undef TO_STREAM_CASE
+#define TO_ROWS_CASE(S)                                                                            \
+    case TO_STREAM_ROWS | S:                                                                       \
+        hipLaunchKernelGGL((k_join_rows<T, SCALED, S>), grid, block, 0, st, d, L.count);           \
+        break;
+                TO_ROWS_CASE(0)
+                TO_ROWS_CASE(1)
+                TO_ROWS_CASE(2)
+                TO_ROWS_CASE(3)
+                TO_ROWS_CASE(4)
+#undef TO_ROWS_CASE
                 default:
                     set_err("stream join: unsupported y chunk");
                     return TO_ERR_INVALID;

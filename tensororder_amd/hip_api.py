@@ -509,7 +509,7 @@ class HipAPI(BaseTensorAPI):
             d.stage = int(v["stage"])
             blob = self._alloc(ctypes.sizeof(d))
             self._upload(blob, bytes(d))
-            variant = v["yc"] | (_lib.TO_STREAM_VEC if v["vec"] else 0)
+            variant = schedule.stream_variant(v, g)
             launches.append(_lib.Launch(op=_lib.TO_OP_JOIN_STREAM, variant=variant, count=1,
                                         grid=v["nblocks"], lds=v["lds"], descs=blob.data_ptr()))
             keep.append(blob)

@@ -219,6 +219,8 @@ STAGE_OUTPUTS = os.environ.get("TENSORORDER_STAGE_OUTPUTS", "1") != "0"
 USE_GEMM = os.environ.get("TENSORORDER_GEMM", "1") != "0"
 # the 16-byte streaming join form (TENSORORDER_STREAM_VEC=0 turns it off)
 STREAM_VEC = os.environ.get("TENSORORDER_STREAM_VEC", "1") != "0"
+# the row form of streamed joins with long contiguous output rows (TENSORORDER_STREAM_ROWS=0)
+STREAM_ROWS = os.environ.get("TENSORORDER_STREAM_ROWS", "1") != "0"
 # float GEMMs with at least this many shared bits accumulate partial sums in double
 WIDE_ACCUM_MIN_SHARED = int(os.environ.get("TENSORORDER_WIDE_ACCUM_MIN_SHARED", "11"))
 GEMM_MIN_FREE = 6
@@ -250,6 +252,13 @@ def gemm_key(esize, mlog, nlog, klog, nb):
 
 def perm_key(esize, rank, nb):
     return "perm/%d/%d/%d" % (esize, rank, int(math.log2(max(1, nb))))
+
+
+def stream_variant(v, g):
+    """Launch variant of a streamed join: YC, TO_STREAM_VEC | YC, or TO_STREAM_ROWS | s."""
+    if v.get("rows"):
+        return _lib.TO_STREAM_ROWS | g["s"]
+    return v["yc"] | (_lib.TO_STREAM_VEC if v["vec"] else 0)
 
 
 def choose_variant(g, nb, esize):
@@ -286,6 +295,12 @@ def choose_variant(g, nb, esize):
                     perm_x=not _canonical(g["xf"], g["xs"]), perm_y=not _canonical(g["yf"], g["ys"]))
     yc = min(16, 1 << fy)
     stage = 0
+    # row form: outputs C[f][y] with 2^fy >= 32 contiguous y and few shared bits, written
+    # as whole rows (each thread keeps 16 bytes of y and their Y values in registers)
+    if STREAM_ROWS and g["shy"] == 0 and g["shf"] == fy and 5 <= fy <= (10 if esize == 4 else 9) \
+            and fx >= 6 and s <= 4:
+        return dict(kind="stream", yc=16, nych=1, lds=0, stage=0, vec=False, rows=True,
+                    nblocks=int(min(nb << (fx - 6), MAX_STREAM_BLOCKS)))
     # 16-byte form: each thread owns 16 bytes of consecutive f (X's lowest bits are free)
     # and stores its 2^fy outputs per f as 16-byte chunks (all y of one f adjacent)
     lvf = 2 if esize == 4 else 1
@@ -665,8 +680,7 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
         d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
         d.lnych = v["nych"].bit_length() - 1
         d.stage = int(v["stage"])
-        variant = v["yc"] | (_lib.TO_STREAM_VEC if v["vec"] else 0)
-        out.append(("stream", _lib.TO_OP_JOIN_STREAM, variant, v["lds"], d, v["nblocks"]))
+        out.append(("stream", _lib.TO_OP_JOIN_STREAM, stream_variant(v, g), v["lds"], d, v["nblocks"]))
         return out
     d.partial = abase + scratch["partial"][0]
     d.nsplit = v["nsplit"]

@@ -324,3 +324,23 @@ def test_gemm_every_tile_variant(hip, hip32, et):
                 base = 3 if v & 2 else 1
                 if v & 16 and base in err:
                     assert e <= 2 * err[base] + 1e-7, (v, e, err[base])
+
+
+@pytest.mark.parametrize("ra,rb,s", [(14, 8, 1), (12, 7, 2), (13, 12, 3), (16, 10, 0), (12, 9, 4), (11, 6, 1)])
+def test_tensordot_row_form(hip, hip32, ra, rb, s):
+    # joins whose outputs are long contiguous rows (the larger operand's free bits high,
+    # 2^fy >= 32 contiguous y) and few shared bits: the row form of the streamed join
+    rng = np.random.default_rng(ra * 31 + rb)
+    _say("case row form", ra, rb, s)
+    a = rng.uniform(0.5, 2.0, (2,) * ra)
+    b = rng.uniform(0.5, 2.0, (2,) * rb)
+    la = [int(x) for x in rng.permutation(ra)[:s]]
+    lb = [int(x) for x in rng.permutation(rb)[:s]]
+    fx, fy = ra - s, rb - s
+    assert fx >= 6 and 5 <= fy <= 10 and s <= 4 and not (fy >= 6 and s >= 4)  # the row form's range
+    want = np.tensordot(a, b, (la, lb))
+    got = hip.tensordot(a, b, (la, lb)).numpy()
+    np.testing.assert_allclose(got, want, rtol=1e-12)
+    got32 = np.asarray(hip32.tensordot(a.astype(np.float32), b.astype(np.float32), (la, lb)).numpy(), np.float64)
+    np.testing.assert_allclose(got32, np.tensordot(a.astype(np.float32).astype(np.float64),
+                                                   b.astype(np.float32).astype(np.float64), (la, lb)), rtol=1e-6)
