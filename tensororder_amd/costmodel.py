@@ -57,7 +57,7 @@ def join_footprint(n, nodes, W, esize):
     v = n.geom["var"]
     if v["kind"] == "splitk":
         total += v["nout"] * v["nsplit"]
-    elif v["kind"] == "gemm":
+    if v["kind"] in ("gemm", "splitk"):
         for key, ch in (("perm_x", nodes[n.left]), ("perm_y", nodes[n.right])):
             if v.get(key):
                 total += (1 << ch.rank) * (nb if ch.batched else 1)

@@ -433,10 +433,17 @@ __global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __re
     if (s >= 8) {
         const int64_t chunk_hi = (1ll << (s - 8)) / nsplit;  // 256-wide steps per split
         const int64_t xlo = pdep(tid, d->xs_pos, 8), ylo = pdep(tid, d->ys_pos, 8);
-        for (int64_t kh = sp * chunk_hi; kh < (sp + 1) * chunk_hi; ++kh) {
-            const int64_t xo = pdep(kh, d->xs_pos + 8, s - 8) + xlo;
-            const int64_t yo = pdep(kh, d->ys_pos + 8, s - 8) + ylo;
-            sum += (double)(X[xo] * T(sx)) * (double)(Y[yo] * T(sy));
+        // the high shared index kh advances by one per step: add one inside the bit masks
+        // of the operands (carries skip the bits outside the mask) instead of a pdep
+        const int64_t xmask = uniform64(pdep(~0ull, d->xs_pos + 8, s - 8));
+        const int64_t ymask = uniform64(pdep(~0ull, d->ys_pos + 8, s - 8));
+        int64_t xh = uniform64(pdep(sp * chunk_hi, d->xs_pos + 8, s - 8));
+        int64_t yh = uniform64(pdep(sp * chunk_hi, d->ys_pos + 8, s - 8));
+#pragma unroll 4
+        for (int64_t kh = 0; kh < chunk_hi; ++kh) {
+            sum += (double)(X[xh + xlo] * T(sx)) * (double)(Y[yh + ylo] * T(sy));
+            xh = ((xh | ~xmask) + 1) & xmask;
+            yh = ((yh | ~ymask) + 1) & ymask;
         }
     } else if (tid < (1 << s)) {
         sum = (double)(X[pdep(tid, d->xs_pos, s)] * T(sx)) * (double)(Y[pdep(tid, d->ys_pos, s)] * T(sy));

@@ -223,6 +223,9 @@ STREAM_VEC = os.environ.get("TENSORORDER_STREAM_VEC", "1") != "0"
 STREAM_ROWS = os.environ.get("TENSORORDER_STREAM_ROWS", "1") != "0"
 # float GEMMs with at least this many shared bits accumulate partial sums in double
 WIDE_ACCUM_MIN_SHARED = int(os.environ.get("TENSORORDER_WIDE_ACCUM_MIN_SHARED", "11"))
+# split-K operands with at least this many shared bits are permuted first when their
+# lowest shared bits are scattered (TENSORORDER_SPLITK_PERMUTE_MIN_SHARED)
+SPLITK_PERMUTE_MIN_SHARED = int(os.environ.get("TENSORORDER_SPLITK_PERMUTE_MIN_SHARED", "14"))
 GEMM_MIN_FREE = 6
 GEMM_MIN_SHARED = 4
 GEMM128_MIN_SHARED = {(8, False): 4, (4, False): 5, (4, True): 5}
@@ -346,8 +349,15 @@ def choose_variant(g, nb, esize):
     # finishing: one thread per output, or one block per output when it has many partials
     fin_wide = nsplit >= 32
     fin_blocks = min(nout, MAX_STREAM_BLOCKS) if fin_wide else min(-(-nout // 256), MAX_STREAM_BLOCKS)
+    # The 256 threads of a block read shared indices k = 0..255 of each operand; an operand
+    # whose 8 lowest shared bits are not its 8 lowest bits in order is read 4 bytes per
+    # cache line, so a long one is first permuted into [free][shared] form (shared bits
+    # lowest, in k order)
+    def scattered(spos):
+        return s >= SPLITK_PERMUTE_MIN_SHARED and list(spos[:8]) != list(range(8))
     return dict(kind="splitk", nsplit=nsplit, nout=nout, nblocks=int(nout * nsplit),
-                fin_blocks=int(fin_blocks), fin_variant=1 if fin_wide else 0)
+                fin_blocks=int(fin_blocks), fin_variant=1 if fin_wide else 0,
+                perm_x=scattered(g["xs"]), perm_y=scattered(g["ys"]))
 
 
 class LeafTable:
@@ -494,14 +504,14 @@ def _pos_array(vals):
 
 
 def _temporaries(arena, n, nodes, W, esize, partial, keep=False):
-    """Scratch buffers a join needs while it runs (split-K partial sums, permuted GEMM
-    operands); allocated after the output and released at once, or returned for the
+    """Scratch buffers a join needs while it runs (split-K partial sums, permuted GEMM and
+    split-K operands); allocated after the output and released at once, or returned for the
     caller to release (``keep``) when other joins run beside it."""
     v = n.geom["var"]
     blocks = []
     if v["kind"] == "splitk":
         blocks.append(("partial", arena.alloc(v["nout"] * v["nsplit"] * 8)))  # double partial sums
-    elif v["kind"] == "gemm":
+    if v["kind"] in ("gemm", "splitk"):
         g = n.geom
         X, Y = (nodes[n.left], nodes[n.right]) if g["x_is_left"] else (nodes[n.right], nodes[n.left])
         for key, op, need in (("perm_x", X, v["perm_x"]), ("perm_y", Y, v["perm_y"])):
@@ -630,24 +640,25 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
     nb = W if n.batched else 1
     X, Y = (nodes[n.left], nodes[n.right]) if g["x_is_left"] else (nodes[n.right], nodes[n.left])
     out = []
+    # operands permuted into [free][shared] form (shared bits lowest, in k order) first
+    operands = []
+    for key, op, fpos, spos in (("perm_x", X, g["xf"], g["xs"]), ("perm_y", Y, g["yf"], g["ys"])):
+        nb_op = W if (n.batched and op.batched) else 1
+        stride = (1 << op.rank) if (n.batched and op.batched) else 0
+        if v.get(key):
+            dst = abase + scratch[key][0]
+            out_pos = [0] * op.rank
+            for t, p in enumerate(spos):
+                out_pos[p] = t
+            for i, p in enumerate(fpos):
+                out_pos[p] = len(spos) + i
+            pd, grid, lds, pvar = perm_desc(op.ptr, dst, op.rank, out_pos, nb_op, 1 << op.rank, esize)
+            out.append(("permute", _lib.TO_OP_PERMUTE, pvar, lds, pd, grid))
+            operands.append((dst, stride, op, True))
+        else:
+            operands.append((op.ptr, stride, op, False))
     if v["kind"] == "gemm":
-        operands = []
-        for key, op, fpos, spos in (("perm_x", X, g["xf"], g["xs"]), ("perm_y", Y, g["yf"], g["ys"])):
-            nb_op = W if (n.batched and op.batched) else 1
-            stride = (1 << op.rank) if (n.batched and op.batched) else 0
-            if v[key]:
-                dst = abase + scratch[key][0]
-                out_pos = [0] * op.rank
-                for t, p in enumerate(spos):
-                    out_pos[p] = t
-                for i, p in enumerate(fpos):
-                    out_pos[p] = len(spos) + i
-                pd, grid, lds, pvar = perm_desc(op.ptr, dst, op.rank, out_pos, nb_op, 1 << op.rank, esize)
-                out.append(("permute", _lib.TO_OP_PERMUTE, pvar, lds, pd, grid))
-                operands.append((dst, stride, op))
-            else:
-                operands.append((op.ptr, stride, op))
-        (a, ab, xo), (b, bb, yo) = operands
+        (a, ab, xo, _), (b, bb, yo, _) = operands
         gd = _lib.GemmDesc()
         gd.a, gd.b, gd.c = a, b, n.ptr
         gd.ab, gd.bb, gd.cb = ab, bb, n.bstride
@@ -666,6 +677,14 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
     d.xb = X.bstride if n.batched else 0
     d.yb = Y.bstride if n.batched else 0
     d.cb = n.bstride
+    xpos = (g["xf"], g["xs"])
+    ypos = (g["yf"], g["ys"])
+    if operands[0][3]:  # X permuted: [free][shared], dense per batch item
+        d.x, d.xb = operands[0][0], operands[0][1]
+        xpos = (list(range(g["s"], g["s"] + g["fx"])), list(range(g["s"])))
+    if operands[1][3]:
+        d.y, d.yb = operands[1][0], operands[1][1]
+        ypos = (list(range(g["s"], g["s"] + g["fy"])), list(range(g["s"])))
     if scaled:
         d.xs, d.ys, d.cs = recp(X), recp(Y), recp(n)
         d.xsb = X.rec_stride if n.batched else 0
@@ -674,8 +693,8 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
     d.shf, d.shy = g["shf"], g["shy"]
     d.nbatch = nb
     d.block0 = 0
-    d.xf_pos, d.xs_pos = _pos_array(g["xf"]), _pos_array(g["xs"])
-    d.yf_pos, d.ys_pos = _pos_array(g["yf"]), _pos_array(g["ys"])
+    d.xf_pos, d.xs_pos = _pos_array(xpos[0]), _pos_array(xpos[1])
+    d.yf_pos, d.ys_pos = _pos_array(ypos[0]), _pos_array(ypos[1])
     if v["kind"] == "stream":
         d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
         d.lnych = v["nych"].bit_length() - 1

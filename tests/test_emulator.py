@@ -59,7 +59,7 @@ def _two_tensor_plan(ra, rb, shared, seed):
 
 
 DENSE = [(12, 12, 12), (13, 11, 10), (9, 14, 9), (6, 6, 0), (10, 3, 1), (12, 12, 6), (14, 11, 5), (11, 15, 5),
-         (10, 10, 4)]
+         (10, 10, 4), (15, 15, 15), (16, 15, 14)]
 
 
 @pytest.mark.parametrize("ra,rb,s", DENSE)
@@ -72,6 +72,9 @@ def test_dense_joins_emulated(ra, rb, s, et, tol):
     got, prog = emulator.emulate_plan(tree, tables, G, total, et, 1)
     if s >= 9 and ra + rb - 2 * s <= 6:
         assert prog.stats["splitk"] >= 1
+    if s >= schedule.SPLITK_PERMUTE_MIN_SHARED:  # scrambled shared bits: operands permuted first
+        kinds = [k for k, _ in prog.prologue_info + prog.wave_info]
+        assert "splitk" in kinds and "permute" in kinds
     if ra - s >= 6 and rb - s >= 6 and s >= 4:
         assert prog.stats["gemm"] >= 1
     assert got == pytest.approx(want, rel=tol, abs=tol * 1e-3 * abs(want))

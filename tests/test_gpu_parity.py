@@ -112,7 +112,8 @@ def test_graph_and_eager_agree(hip):
 
 
 @pytest.mark.parametrize("ra,rb,s", [(12, 12, 6), (14, 11, 5), (11, 15, 5), (10, 10, 4), (13, 11, 10),
-                                     (12, 12, 12), (13, 13, 5), (14, 13, 6), (16, 14, 4), (18, 17, 11)])
+                                     (12, 12, 12), (13, 13, 5), (14, 13, 6), (16, 14, 4), (18, 17, 11),
+                                     (15, 15, 15), (16, 15, 14), (22, 22, 22)])
 def test_dense_join_plans(hip, hip32, ra, rb, s):
     from test_emulator import _two_tensor_plan
 
