@@ -61,7 +61,7 @@ def kernel_symbol(L, scaled):
         return "k_gemm<%s, %s, %s>" % (T, S, "true" if v & 2 else "false")
     if op == _lib.TO_OP_JOIN_STREAM:
         if v & _lib.TO_STREAM_ROWS:
-            return "k_join_rows<%s, %s, %d>" % (T, S, v & 0xff)
+            return "k_join_rows<%s, %s>" % (T, S)
         if v & _lib.TO_STREAM_VEC:
             return "k_join_stream_vec<%s, %s, %d>" % (T, S, v & ~_lib.TO_STREAM_VEC)
         return "k_join_stream<%s, %s, %d>" % (T, S, v)

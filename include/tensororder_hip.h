@@ -51,7 +51,7 @@ extern "C" {
                                 or TO_STREAM_VEC | YC for the 16-byte form: fx >= 10 (float) /
                                 9 (double), the lowest 2 / 1 bits of X free, shy == 0,
                                 shf == fy, YC == 2^fy, LDS 8*2^s (16-aligned) + YC*2^s entries;
-                                or TO_STREAM_ROWS | s for the row form: shy == 0, shf == fy,
+                                or TO_STREAM_ROWS for the row form: shy == 0, shf == fy,
                                 5 <= fy <= 10 (float) / 9 (double), fx >= 6, s <= 4, no LDS */
 #define TO_STREAM_VEC 0x100
 #define TO_STREAM_ROWS 0x200

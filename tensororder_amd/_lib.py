@@ -27,7 +27,7 @@ TO_F64, TO_F32S = 0, 1
     TO_OP_BATCH_BEGIN,
 ) = range(1, 13)
 TO_STREAM_VEC = 0x100  # stream variant bit: the 16-byte form
-TO_STREAM_ROWS = 0x200  # stream variant bit: the row form (low bits: s)
+TO_STREAM_ROWS = 0x200  # stream variant: the row form
 MAX_BITS = 32
 ABI_VERSION = 3
 

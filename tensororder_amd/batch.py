@@ -43,7 +43,7 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
         for n in nodes:
             if not n.is_leaf:
                 g = schedule.join_geometry(nodes[n.left].rank, nodes[n.right].rank, n.lmap, n.rmap)
-                g["var"] = schedule.choose_variant(g, 1, esize)
+                g["var"] = schedule.choose_variant(g, 1, esize, launch_bound=True)
                 n.geom = g
         peak, scratch = schedule.plan_memory(nodes, tables, 1, esize)
         per.append(dict(nodes=nodes, tables=tables, base=arena_top, scratch=scratch, peak=peak))

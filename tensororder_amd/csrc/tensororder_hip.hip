@@ -433,17 +433,29 @@ __global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __re
     if (s >= 8) {
         const int64_t chunk_hi = (1ll << (s - 8)) / nsplit;  // 256-wide steps per split
         const int64_t xlo = pdep(tid, d->xs_pos, 8), ylo = pdep(tid, d->ys_pos, 8);
-        // the high shared index kh advances by one per step: add one inside the bit masks
-        // of the operands (carries skip the bits outside the mask) instead of a pdep
-        const int64_t xmask = uniform64(pdep(~0ull, d->xs_pos + 8, s - 8));
-        const int64_t ymask = uniform64(pdep(~0ull, d->ys_pos + 8, s - 8));
-        int64_t xh = uniform64(pdep(sp * chunk_hi, d->xs_pos + 8, s - 8));
-        int64_t yh = uniform64(pdep(sp * chunk_hi, d->ys_pos + 8, s - 8));
+        // When the high shared bits of both operands lie in ascending positions (always so
+        // after a permute to [free][shared] form), pdep(kh + 1) is pdep(kh) plus one inside
+        // the bit mask (carries skip the bits outside it), so the step needs no pdep loop
+        bool ascending = true;
+        for (int i = 9; i < s; ++i)
+            ascending = ascending && d->xs_pos[i - 1] < d->xs_pos[i] && d->ys_pos[i - 1] < d->ys_pos[i];
+        if (ascending) {
+            const int64_t xmask = uniform64(pdep(~0ull, d->xs_pos + 8, s - 8));
+            const int64_t ymask = uniform64(pdep(~0ull, d->ys_pos + 8, s - 8));
+            int64_t xh = uniform64(pdep(sp * chunk_hi, d->xs_pos + 8, s - 8));
+            int64_t yh = uniform64(pdep(sp * chunk_hi, d->ys_pos + 8, s - 8));
 #pragma unroll 4
-        for (int64_t kh = 0; kh < chunk_hi; ++kh) {
-            sum += (double)(X[xh + xlo] * T(sx)) * (double)(Y[yh + ylo] * T(sy));
-            xh = ((xh | ~xmask) + 1) & xmask;
-            yh = ((yh | ~ymask) + 1) & ymask;
+            for (int64_t kh = 0; kh < chunk_hi; ++kh) {
+                sum += (double)(X[xh + xlo] * T(sx)) * (double)(Y[yh + ylo] * T(sy));
+                xh = ((xh | ~xmask) + 1) & xmask;
+                yh = ((yh | ~ymask) + 1) & ymask;
+            }
+        } else {
+            for (int64_t kh = sp * chunk_hi; kh < (sp + 1) * chunk_hi; ++kh) {
+                const int64_t xo = pdep(kh, d->xs_pos + 8, s - 8) + xlo;
+                const int64_t yo = pdep(kh, d->ys_pos + 8, s - 8) + ylo;
+                sum += (double)(X[xo] * T(sx)) * (double)(Y[yo] * T(sy));
+            }
         }
     } else if (tid < (1 << s)) {
         sum = (double)(X[pdep(tid, d->xs_pos, s)] * T(sx)) * (double)(Y[pdep(tid, d->ys_pos, s)] * T(sy));
@@ -769,16 +781,17 @@ __global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* 
 
 // ---------------------------------------------------------------------------------
 // streaming join, row form: outputs C[f][y] with the 2^fy values of y contiguous
-// (shy == 0, shf == fy, fy >= 5) and few shared bits (s = S <= 4) -- an outer product of
+// (shy == 0, shf == fy, fy >= 5) and few shared bits (s <= 4) -- an outer product of
 // long rows, bound by the output writes.  Each thread owns VY = 16 bytes of consecutive y
 // of a row and keeps their Y[y][k] (VY * 2^S values) in registers for its batch item;
 // the threads of one row read the same X[f][k] (one load per wave), form VY outputs and
 // write them as one 16-byte store, so a wave writes whole rows.  A block walks a
 // contiguous range of tiles of 64 rows of one batch item.
 
-template <typename T, bool SCALED, int S>
+template <typename T, bool SCALED>
 __global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __restrict__ descs, int ndesc) {
-    constexpr int K = 1 << S;
+    constexpr int KMAX = 16;  // s <= 4; one kernel for every s, so joins of one batch step
+                              // with different s share a launch
     constexpr int VY = 16 / sizeof(T);
     constexpr int ROWS = 64;  // rows per tile
     typedef typename Vec16<T>::type V;
@@ -788,12 +801,13 @@ __global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __rest
     const int tpr = (1 << fy) / VY;          // threads per row (the host keeps it <= kBlock)
     const int rb = kBlock / tpr;             // rows per pass
     const int j = tid % tpr, r = tid / tpr;  // y vector and row of this thread
+    const int S = d->s, K = 1 << S;
     const TO_GLOBAL T* __restrict__ X = gptr<const T>(d->x);
     const TO_GLOBAL T* __restrict__ Y = gptr<const T>(d->y);
     TO_GLOBAL T* __restrict__ C = gptr<T>(d->c);
-    int64_t kx[K];
+    int64_t kx[KMAX];
 #pragma unroll
-    for (int k = 0; k < K; ++k) kx[k] = uniform64(pdep(k, d->xs_pos, S));
+    for (int k = 0; k < KMAX; ++k) kx[k] = k < K ? uniform64(pdep(k, d->xs_pos, S)) : 0;
     const int local = blockIdx.x - d->block0;
     const int tb = fx - 6;                   // tile bits of f
     const int64_t nft = 1ll << tb;
@@ -801,7 +815,7 @@ __global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __rest
     const int64_t t0 = ntiles * local / d->nblocks, t1 = ntiles * (local + 1) / d->nblocks;
     const int64_t fmask = pdep(~0ull, d->xf_pos, fx);
     const int64_t xstep = pdep((uint64_t)rb, d->xf_pos, fx);  // rb rows on, in X's bit layout
-    T yr[K][VY];
+    T yr[KMAX][VY];
     int64_t staged = -1;
     float sxy = 1.0f;
     int64_t exy = 0;
@@ -821,10 +835,12 @@ __global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __rest
             }
             const TO_GLOBAL T* Yb = Y + batch * d->yb;
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const int64_t ky = pdep(k, d->ys_pos, S);
+            for (int k = 0; k < KMAX; ++k) {
+                if (k < K) {
+                    const int64_t ky = pdep(k, d->ys_pos, S);
 #pragma unroll
-                for (int e = 0; e < VY; ++e) yr[k][e] = Yb[pdep(j * VY + e, d->yf_pos, fy) + ky];
+                    for (int e = 0; e < VY; ++e) yr[k][e] = Yb[pdep(j * VY + e, d->yf_pos, fy) + ky];
+                }
             }
             staged = batch;
         }
@@ -834,15 +850,16 @@ __global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __rest
         const int64_t f0 = ft * ROWS + r;
         int64_t xo = pdep((uint64_t)f0, d->xf_pos, fx);
         for (int64_t f = f0; f < ft * ROWS + ROWS; f += rb) {
-            T xv[K];
+            T xv[KMAX];
 #pragma unroll
-            for (int k = 0; k < K; ++k) xv[k] = Xb[xo + kx[k]];
+            for (int k = 0; k < KMAX; ++k) xv[k] = k < K ? Xb[xo + kx[k]] : T(0);
             V o;
 #pragma unroll
             for (int e = 0; e < VY; ++e) {
                 T a = T(0);
 #pragma unroll
-                for (int k = 0; k < K; ++k) a = fma(xv[k], yr[k][e], a);
+                for (int k = 0; k < KMAX; ++k)
+                    if (k < K) a = fma(xv[k], yr[k][e], a);
                 o[e] = SCALED ? a * T(sxy) : a;
                 if (SCALED) mrun = fmaxf(mrun, (float)absval(o[e]));
             }
@@ -1846,16 +1863,9 @@ int launch_one(const to_launch& L, hipStream_t st) {
                 TO_STREAM_CASE(8)
                 TO_STREAM_CASE(16)
 #undef TO_STREAM_CASE
-#define TO_ROWS_CASE(S)                                                                            \
-    case TO_STREAM_ROWS | S:                                                                       \
-        hipLaunchKernelGGL((k_join_rows<T, SCALED, S>), grid, block, 0, st, d, L.count);           \
-        break;
-                TO_ROWS_CASE(0)
-                TO_ROWS_CASE(1)
-                TO_ROWS_CASE(2)
-                TO_ROWS_CASE(3)
-                TO_ROWS_CASE(4)
-#undef TO_ROWS_CASE
+                case TO_STREAM_ROWS:
+                    hipLaunchKernelGGL((k_join_rows<T, SCALED>), grid, block, 0, st, d, L.count);
+                    break;
                 default:
                     set_err("stream join: unsupported y chunk");
                     return TO_ERR_INVALID;

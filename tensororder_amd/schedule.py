@@ -258,14 +258,17 @@ def perm_key(esize, rank, nb):
 
 
 def stream_variant(v, g):
-    """Launch variant of a streamed join: YC, TO_STREAM_VEC | YC, or TO_STREAM_ROWS | s."""
+    """Launch variant of a streamed join: YC, TO_STREAM_VEC | YC, or TO_STREAM_ROWS."""
     if v.get("rows"):
-        return _lib.TO_STREAM_ROWS | g["s"]
+        return _lib.TO_STREAM_ROWS
     return v["yc"] | (_lib.TO_STREAM_VEC if v["vec"] else 0)
 
 
-def choose_variant(g, nb, esize):
-    """Kernel variant and launch shape for a join with ``nb`` batch items."""
+def choose_variant(g, nb, esize, launch_bound=False):
+    """Kernel variant and launch shape for a join with ``nb`` batch items.  With
+    ``launch_bound`` (the many-instance batch, whose steps group the joins of all instances
+    by kernel variant, one launch each) a small join keeps the general kernel rather than a
+    faster form that would add a launch to its step."""
     fx, fy, s = g["fx"], g["fy"], g["s"]
     K = 1 << s
     if USE_GEMM and fx >= GEMM_MIN_FREE and fy >= GEMM_MIN_FREE and s >= GEMM_MIN_SHARED:
@@ -301,7 +304,7 @@ def choose_variant(g, nb, esize):
     # row form: outputs C[f][y] with 2^fy >= 32 contiguous y and few shared bits, written
     # as whole rows (each thread keeps 16 bytes of y and their Y values in registers)
     if STREAM_ROWS and g["shy"] == 0 and g["shf"] == fy and 5 <= fy <= (10 if esize == 4 else 9) \
-            and fx >= 6 and s <= 4:
+            and fx >= 6 and s <= 4 and not (launch_bound and fx + fy + math.log2(max(nb, 1)) < 20):
         return dict(kind="stream", yc=16, nych=1, lds=0, stage=0, vec=False, rows=True,
                     nblocks=int(min(nb << (fx - 6), MAX_STREAM_BLOCKS)))
     # 16-byte form: each thread owns 16 bytes of consecutive f (X's lowest bits are free)
@@ -354,7 +357,8 @@ def choose_variant(g, nb, esize):
     # cache line, so a long one is first permuted into [free][shared] form (shared bits
     # lowest, in k order)
     def scattered(spos):
-        return s >= SPLITK_PERMUTE_MIN_SHARED and list(spos[:8]) != list(range(8))
+        return s >= (max(18, SPLITK_PERMUTE_MIN_SHARED) if launch_bound else SPLITK_PERMUTE_MIN_SHARED) \
+            and list(spos[:8]) != list(range(8))
     return dict(kind="splitk", nsplit=nsplit, nout=nout, nblocks=int(nout * nsplit),
                 fin_blocks=int(fin_blocks), fin_variant=1 if fin_wide else 0,
                 perm_x=scattered(g["xs"]), perm_y=scattered(g["ys"]))

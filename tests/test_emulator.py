@@ -59,7 +59,7 @@ def _two_tensor_plan(ra, rb, shared, seed):
 
 
 DENSE = [(12, 12, 12), (13, 11, 10), (9, 14, 9), (6, 6, 0), (10, 3, 1), (12, 12, 6), (14, 11, 5), (11, 15, 5),
-         (10, 10, 4), (15, 15, 15), (16, 15, 14)]
+         (10, 10, 4), (15, 15, 15), (16, 15, 14), (18, 17, 12)]
 
 
 @pytest.mark.parametrize("ra,rb,s", DENSE)
