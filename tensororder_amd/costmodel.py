@@ -39,8 +39,8 @@ def join_features(n, nodes, W, esize):
     X, Y = (nodes[n.left], nodes[n.right]) if g["x_is_left"] else (nodes[n.right], nodes[n.left])
     b, f = schedule.join_traffic(n, nodes, W, esize)
     kind = v["kind"]
-    perms = (int(v.get("perm_x", False)) + int(v.get("perm_y", False))) if kind == "gemm" else 0
-    variant = {"stream": math.log2(v.get("yc", 1)) + (5 if v.get("vec") else 0),
+    perms = (int(v.get("perm_x", False)) + int(v.get("perm_y", False))) if kind in ("gemm", "splitk") else 0
+    variant = {"stream": 10 if v.get("rows") else math.log2(v.get("yc", 1)) + (5 if v.get("vec") else 0),
                "gemm": v.get("tile", 0) & 7}.get(kind, 0)
     return [X.rank, Y.rank, n.rank, g["s"], g["fx"], g["fy"], math.log2(max(f, 1)), math.log2(max(b, 1)),
             math.log2(nb), kind == "stream", kind == "splitk", kind == "gemm", esize / 8.0, perms,
