@@ -55,6 +55,8 @@ extern "C" {
                                 5 <= fy <= 10 (float) / 9 (double), fx >= 6, s <= 4, no LDS */
 #define TO_STREAM_VEC 0x100
 #define TO_STREAM_ROWS 0x200
+#define TO_STREAM_ANY 0x400  /* one launch of streamed joins of mixed variants, each given by
+                                its descriptor's kvar; lds = the largest any of them needs */
 #define TO_OP_JOIN_SPLITK 3  /* descs: to_join_desc[count]; partial sums (double) to d.partial */
 #define TO_OP_JOIN_FINISH 4  /* descs: to_join_desc[count]; sums d.partial into C; variant 1:
                                 one block per output (for nsplit >= 32) */
@@ -102,6 +104,8 @@ typedef struct to_join_desc {
                                   shf == fy, one y chunk; 256 * 2^fy entries after the Y
                                   slab); bit 1: X tile staged (all shared bits of X below
                                   bit 8 + s; 256 * 2^s entries after that) */
+    int32_t kvar;              /* stream: this join's own variant (YC, TO_STREAM_VEC | YC or
+                                  TO_STREAM_ROWS) when the launch variant is TO_STREAM_ANY */
     int8_t xf_pos[TO_MAX_BITS];
     int8_t xs_pos[TO_MAX_BITS];
     int8_t yf_pos[TO_MAX_BITS];

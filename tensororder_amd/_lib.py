@@ -28,8 +28,9 @@ TO_F64, TO_F32S = 0, 1
 ) = range(1, 13)
 TO_STREAM_VEC = 0x100  # stream variant bit: the 16-byte form
 TO_STREAM_ROWS = 0x200  # stream variant: the row form
+TO_STREAM_ANY = 0x400  # stream launch variant: mixed joins, each by its descriptor's kvar
 MAX_BITS = 32
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 c_i8x32 = ctypes.c_int8 * MAX_BITS
 
@@ -46,7 +47,7 @@ class JoinDesc(ctypes.Structure):
         ("nbatch", ctypes.c_int32),
         ("block0", ctypes.c_int32), ("nblocks", ctypes.c_int32),
         ("nych", ctypes.c_int32), ("nsplit", ctypes.c_int32),
-        ("lnych", ctypes.c_int32), ("stage", ctypes.c_int32),
+        ("lnych", ctypes.c_int32), ("stage", ctypes.c_int32), ("kvar", ctypes.c_int32),
         ("xf_pos", c_i8x32), ("xs_pos", c_i8x32), ("yf_pos", c_i8x32), ("ys_pos", c_i8x32),
     ]
 

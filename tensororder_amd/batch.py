@@ -12,6 +12,8 @@ the device clock, which gives per-instance completion times within the batch.
 import ctypes
 import math
 
+import os
+
 import numpy as np
 
 from . import _lib, schedule
@@ -19,6 +21,8 @@ from ._lib import Launch
 
 ALIGN = schedule.ALIGN
 KIND_ORDER = schedule.KIND_ORDER
+# one launch for all streamed joins of a step (TENSORORDER_BATCH_MERGE_STREAM=0: one per form)
+MERGE_STREAM_FORMS = os.environ.get("TENSORORDER_BATCH_MERGE_STREAM", "1") != "0"
 
 
 class BatchProgram:
@@ -120,6 +124,10 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
         for k, n in enumerate(joins):
             for kind, op, variant, lds, st, nblk in schedule.join_records(
                     n, nodes, 1, esize, scaled, abase, scratch.get(n.idx, {}), recp):
+                # the streamed joins of one step share one launch whatever their form
+                # (each descriptor names its own); the step is bound by launch count
+                if op == _lib.TO_OP_JOIN_STREAM and MERGE_STREAM_FORMS:
+                    variant = _lib.TO_STREAM_ANY
                 key = (KIND_ORDER[kind], op, variant)
                 if kind == "permute":
                     work = (2 * esize * (int(st.nbatch) << int(st.rank)), 0)

@@ -24,7 +24,7 @@
 
 #include "tensororder_hip.h"
 
-#define TO_ABI_VERSION 3
+#define TO_ABI_VERSION 4
 
 namespace {
 
@@ -188,9 +188,8 @@ __global__ __launch_bounds__(kBlock) void k_leaf_gather(const to_leaf_desc* __re
 // Y slab ylds[YC][2^s] of the current (batch item, y chunk).
 
 template <typename T, bool SCALED, int YC>
-__global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __restrict__ descs, int ndesc) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+__device__ __forceinline__ void join_stream_body(const to_join_desc* __restrict__ d, const int local,
+                                                 unsigned char* smem) {
     const int tid = threadIdx.x;
     const int fx = d->fx, fy = d->fy, s = d->s;
     const int K = 1 << s;
@@ -213,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
     }
     __syncthreads();
 
-    const int local = blockIdx.x - d->block0;
+    
     const int nblk = d->nblocks;
 
     if (fx >= 8) {
@@ -404,6 +403,13 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
             }
         }
     }
+}
+
+template <typename T, bool SCALED, int YC>
+__global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __restrict__ descs, int ndesc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    join_stream_body<T, SCALED, YC>(d, blockIdx.x - d->block0, smem);
 }
 
 // ---------------------------------------------------------------------------------
@@ -665,12 +671,11 @@ struct Vec16<double> {
 // tiles of 256 * VF consecutive f of one batch item.
 
 template <typename T, bool SCALED, int YC>
-__global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* __restrict__ descs, int ndesc) {
+__device__ __forceinline__ void join_stream_vec_body(const to_join_desc* __restrict__ d, const int local,
+                                                     unsigned char* smem) {
     constexpr int VF = 16 / sizeof(T);
     constexpr int LVF = sizeof(T) == 4 ? 2 : 1;
     typedef typename Vec16<T>::type V;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
     const int tid = threadIdx.x;
     const int fx = d->fx, fy = d->fy, s = d->s;
     const int K = 1 << s;
@@ -689,7 +694,6 @@ __global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* 
     }
     __syncthreads();
 
-    const int local = blockIdx.x - d->block0;
     const int nblk = d->nblocks;
     const int tb = 8 + LVF;  // f bits inside a tile
     const int hb = fx - tb;
@@ -779,6 +783,13 @@ __global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* 
     }
 }
 
+template <typename T, bool SCALED, int YC>
+__global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* __restrict__ descs, int ndesc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    join_stream_vec_body<T, SCALED, YC>(d, blockIdx.x - d->block0, smem);
+}
+
 // ---------------------------------------------------------------------------------
 // streaming join, row form: outputs C[f][y] with the 2^fy values of y contiguous
 // (shy == 0, shf == fy, fy >= 5) and few shared bits (s <= 4) -- an outer product of
@@ -789,13 +800,12 @@ __global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* 
 // contiguous range of tiles of 64 rows of one batch item.
 
 template <typename T, bool SCALED>
-__global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __restrict__ descs, int ndesc) {
+__device__ __forceinline__ void join_rows_body(const to_join_desc* __restrict__ d, const int local) {
     constexpr int KMAX = 16;  // s <= 4; one kernel for every s, so joins of one batch step
                               // with different s share a launch
     constexpr int VY = 16 / sizeof(T);
     constexpr int ROWS = 64;  // rows per tile
     typedef typename Vec16<T>::type V;
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
     const int tid = threadIdx.x;
     const int fx = d->fx, fy = d->fy;
     const int tpr = (1 << fy) / VY;          // threads per row (the host keeps it <= kBlock)
@@ -808,7 +818,6 @@ __global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __rest
     int64_t kx[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) kx[k] = k < K ? uniform64(pdep(k, d->xs_pos, S)) : 0;
-    const int local = blockIdx.x - d->block0;
     const int tb = fx - 6;                   // tile bits of f
     const int64_t nft = 1ll << tb;
     const int64_t ntiles = (int64_t)d->nbatch << tb;
@@ -870,6 +879,41 @@ __global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __rest
     if (SCALED && staged >= 0) {
         const float w = wave_max(mrun);
         if ((tid & 63) == 0) note_max(d->cs, staged, w);
+    }
+}
+
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __restrict__ descs, int ndesc) {
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    join_rows_body<T, SCALED>(d, blockIdx.x - d->block0);
+}
+
+// Streamed joins of mixed variants in one launch (the many-instance batch groups the joins
+// of one step this way): each block runs the form its descriptor's kvar names.
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_join_stream_any(const to_join_desc* __restrict__ descs, int ndesc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int local = blockIdx.x - d->block0;
+    switch (d->kvar) {
+#define TO_ANY_CASE(YC)                                                  \
+    case YC:                                                             \
+        join_stream_body<T, SCALED, YC>(d, local, smem);                 \
+        break;                                                           \
+    case TO_STREAM_VEC | YC:                                             \
+        join_stream_vec_body<T, SCALED, YC>(d, local, smem);             \
+        break;
+        TO_ANY_CASE(1)
+        TO_ANY_CASE(2)
+        TO_ANY_CASE(4)
+        TO_ANY_CASE(8)
+        TO_ANY_CASE(16)
+#undef TO_ANY_CASE
+        case TO_STREAM_ROWS:
+            join_rows_body<T, SCALED>(d, local);
+            break;
+        default:
+            break;  // rejected on the host (schedule.stream_variant)
     }
 }
 
@@ -1865,6 +1909,9 @@ int launch_one(const to_launch& L, hipStream_t st) {
 #undef TO_STREAM_CASE
                 case TO_STREAM_ROWS:
                     hipLaunchKernelGGL((k_join_rows<T, SCALED>), grid, block, 0, st, d, L.count);
+                    break;
+                case TO_STREAM_ANY:
+                    hipLaunchKernelGGL((k_join_stream_any<T, SCALED>), grid, block, L.lds, st, d, L.count);
                     break;
                 default:
                     set_err("stream join: unsupported y chunk");

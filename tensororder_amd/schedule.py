@@ -290,6 +290,11 @@ def choose_variant(g, nb, esize, launch_bound=False):
             db = bool(tuned["variant"] & 8)
             x6 = bool(tuned["variant"] & 16)
             cap = tuned.get("block_cap") or MAX_GEMM_BLOCKS
+        if launch_bound:
+            # one product form (64x64 tiles, float or wide accumulators) for every product
+            # of a batch step, so that they share a launch
+            big = db = x6 = False
+            cap = MAX_GEMM_BLOCKS
         tn = 6 if (wide and not db) else 7
         tiles = nb << ((fx - 7 + fy - tn) if big else (fx - 6 + fy - 6))
         # float outputs whose columns are contiguous (shy == 0) are written 16 bytes at a
@@ -703,7 +708,8 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
         d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
         d.lnych = v["nych"].bit_length() - 1
         d.stage = int(v["stage"])
-        out.append(("stream", _lib.TO_OP_JOIN_STREAM, stream_variant(v, g), v["lds"], d, v["nblocks"]))
+        d.kvar = stream_variant(v, g)
+        out.append(("stream", _lib.TO_OP_JOIN_STREAM, d.kvar, v["lds"], d, v["nblocks"]))
         return out
     d.partial = abase + scratch["partial"][0]
     d.nsplit = v["nsplit"]

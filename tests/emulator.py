@@ -154,8 +154,12 @@ class Emulator:
                              maxbits=int(np.array([m], np.float32).view(np.uint32)[0]))
 
     def op_2(self, L):
+        valid = {y | f for y in (1, 2, 4, 8, 16) for f in (0, _lib.TO_STREAM_VEC)} | {_lib.TO_STREAM_ROWS}
         for i in range(L.count):
-            self._join(self.heap.read_struct(_lib.JoinDesc, L.descs + i * ctypes.sizeof(_lib.JoinDesc)))
+            d = self.heap.read_struct(_lib.JoinDesc, L.descs + i * ctypes.sizeof(_lib.JoinDesc))
+            # a launch of one form carries it in every descriptor; a mixed launch names each
+            assert d.kvar in valid and (L.variant == _lib.TO_STREAM_ANY or d.kvar == L.variant)
+            self._join(d)
 
     def op_3(self, L):  # split-K partials: the emulator computes the join at FINISH
         pass
