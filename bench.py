@@ -60,6 +60,8 @@ def kernel_symbol(L, scaled):
             return "k_gemm128<%s, %s, %s, %s>" % (T, S, wide, tr)
         return "k_gemm<%s, %s, %s>" % (T, S, "true" if v & 2 else "false")
     if op == _lib.TO_OP_JOIN_STREAM:
+        if v & _lib.TO_STREAM_ANY:
+            return "k_join_stream_any<%s, %s>" % (T, S)
         if v & _lib.TO_STREAM_ROWS:
             return "k_join_rows<%s, %s>" % (T, S)
         if v & _lib.TO_STREAM_VEC:

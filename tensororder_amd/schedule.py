@@ -219,6 +219,9 @@ STAGE_OUTPUTS = os.environ.get("TENSORORDER_STAGE_OUTPUTS", "1") != "0"
 USE_GEMM = os.environ.get("TENSORORDER_GEMM", "1") != "0"
 # the 16-byte streaming join form (TENSORORDER_STREAM_VEC=0 turns it off)
 STREAM_VEC = os.environ.get("TENSORORDER_STREAM_VEC", "1") != "0"
+# streamed joins of one level moving fewer bytes than this share one launch of the
+# mixed-form kernel (TENSORORDER_STREAM_MERGE_BYTES=0 turns it off)
+STREAM_MERGE_BYTES = int(os.environ.get("TENSORORDER_STREAM_MERGE_BYTES", str(4 << 20)))
 # the row form of streamed joins with long contiguous output rows (TENSORORDER_STREAM_ROWS=0)
 STREAM_ROWS = os.environ.get("TENSORORDER_STREAM_ROWS", "1") != "0"
 # float GEMMs with at least this many shared bits accumulate partial sums in double
@@ -861,6 +864,10 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
                     work = join_traffic(n, nodes, W, esize)
                 else:
                     work = (0, 0)
+                # small streamed joins of one level share one launch whatever their form
+                # (each descriptor names its own): they are bound by launch latency
+                if kind == "stream" and work[0] < STREAM_MERGE_BYTES:
+                    variant = _lib.TO_STREAM_ANY
                 key = (KIND_ORDER[kind], op, variant)
                 groups.setdefault(key, []).append((kind, st, nblk, lds, n.idx, work))
         for key in sorted(groups):
