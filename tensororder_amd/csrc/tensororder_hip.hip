@@ -751,18 +751,58 @@ __device__ __forceinline__ void join_stream_vec_body(const to_join_desc* __restr
             }
         }
         // outputs (f, y) of this thread: one run of VF * YC entries at f0 * YC
-        auto Cp = C + batch * cb + ((((ft << tb) | ((int64_t)tid << LVF))) * YC);
+        constexpr int RUN = VF * YC;
         float m = 0.0f;
+        if constexpr (RUN * sizeof(T) >= 128) {
+            // Runs of >= 128 bytes: direct 16-byte stores of one instruction would land a
+            // run apart.  The wave's runs (consecutive in C) go through its 4 KB of LDS, L
+            // lanes' runs per round, and each round leaves as 4 stores of 16 bytes per lane
+            // over 1 KB of consecutive bytes.
+            constexpr int E = 4096 / sizeof(T);  // entries per round
+            constexpr int L = E / RUN;           // lanes per round
+            const int lane = tid & 63;
+            T* st = reinterpret_cast<T*>(smem + ((8 * K + 15) & ~15) + (size_t)YC * K * sizeof(T)) + (tid >> 6) * E;
+            auto Cw = C + batch * cb + ((((ft << tb) | ((int64_t)(tid & ~63) << LVF))) * YC);
 #pragma unroll
-        for (int j = 0; j < VF * YC; j += VF) {
-            V o;
+            for (int q = 0; q < 64 / L; ++q) {
+                if (lane / L == q) {
 #pragma unroll
-            for (int c = 0; c < VF; ++c) {
-                const T v = acc[(j + c) / YC][(j + c) % YC];
-                o[c] = v;
-                if (SCALED) m = fmaxf(m, (float)absval(v));
+                    for (int j = 0; j < RUN; j += VF) {
+                        V o;
+#pragma unroll
+                        for (int c = 0; c < VF; ++c) {
+                            const T v = acc[(j + c) / YC][(j + c) % YC];
+                            o[c] = v;
+                            if (SCALED) m = fmaxf(m, (float)absval(v));
+                        }
+                        *reinterpret_cast<V*>(st + (lane % L) * RUN + j) = o;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+                for (int r = 0; r < E / (64 * VF); ++r) {
+                    const int e = (r * 64 + lane) * VF;
+                    *reinterpret_cast<TO_GLOBAL V*>(Cw + q * E + e) = *reinterpret_cast<const V*>(st + e);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             }
-            *reinterpret_cast<TO_GLOBAL V*>(Cp + j) = o;
+        } else {
+            auto Cp = C + batch * cb + ((((ft << tb) | ((int64_t)tid << LVF))) * YC);
+#pragma unroll
+            for (int j = 0; j < RUN; j += VF) {
+                V o;
+#pragma unroll
+                for (int c = 0; c < VF; ++c) {
+                    const T v = acc[(j + c) / YC][(j + c) % YC];
+                    o[c] = v;
+                    if (SCALED) m = fmaxf(m, (float)absval(v));
+                }
+                *reinterpret_cast<TO_GLOBAL V*>(Cp + j) = o;
+            }
         }
         if (SCALED) {
             mrun = fmaxf(mrun, m);
