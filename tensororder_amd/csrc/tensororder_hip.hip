@@ -753,13 +753,14 @@ __device__ __forceinline__ void join_stream_vec_body(const to_join_desc* __restr
         // outputs (f, y) of this thread: one run of VF * YC entries at f0 * YC
         constexpr int RUN = VF * YC;
         float m = 0.0f;
-        if constexpr (RUN * sizeof(T) >= 128) {
-            // Runs of >= 128 bytes: direct 16-byte stores of one instruction would land a
+        if constexpr (RUN * sizeof(T) >= 64) {
+            // Runs of >= 64 bytes: direct 16-byte stores of one instruction would land a
             // run apart.  The wave's runs (consecutive in C) go through its 4 KB of LDS, L
             // lanes' runs per round, and each round leaves as 4 stores of 16 bytes per lane
             // over 1 KB of consecutive bytes.
             constexpr int E = 4096 / sizeof(T);  // entries per round
             constexpr int L = E / RUN;           // lanes per round
+            static_assert(L >= 1 && L <= 64 && 64 % L == 0, "a round stages whole runs of one wave");
             const int lane = tid & 63;
             T* st = reinterpret_cast<T*>(smem + ((8 * K + 15) & ~15) + (size_t)YC * K * sizeof(T)) + (tid >> 6) * E;
             auto Cw = C + batch * cb + ((((ft << tb) | ((int64_t)(tid & ~63) << LVF))) * YC);

@@ -323,8 +323,8 @@ def choose_variant(g, nb, esize, launch_bound=False):
         lds = ((8 * K + 15) & ~15) + (1 << fy) * K * esize
         if lds <= LDS_LIMIT:
             ntiles = nb << (fx - 8 - lvf)
-            # runs of >= 128 bytes per thread leave through 4 KB of LDS per wave
-            if (16 // esize) * (1 << fy) * esize >= 128:
+            # runs of >= 64 bytes per thread leave through 4 KB of LDS per wave
+            if (16 // esize) * (1 << fy) * esize >= 64:
                 lds += 4 * 4096
             return dict(kind="stream", yc=1 << fy, nych=1, lds=lds, stage=0, vec=True,
                         nblocks=int(min(ntiles, MAX_STREAM_BLOCKS)))
