@@ -838,14 +838,14 @@ __global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* 
 // of a row and keeps their Y[y][k] (VY * 2^S values) in registers for its batch item;
 // the threads of one row read the same X[f][k] (one load per wave), form VY outputs and
 // write them as one 16-byte store, so a wave writes whole rows.  A block walks a
-// contiguous range of tiles of 64 rows of one batch item.
+// contiguous range of tiles (64 rows, or 8 passes of rows if fewer) of one batch item.
 
 template <typename T, bool SCALED>
 __device__ __forceinline__ void join_rows_body(const to_join_desc* __restrict__ d, const int local) {
     constexpr int KMAX = 16;  // s <= 4; one kernel for every s, so joins of one batch step
                               // with different s share a launch
     constexpr int VY = 16 / sizeof(T);
-    constexpr int ROWS = 64;  // rows per tile
+
     typedef typename Vec16<T>::type V;
     const int tid = threadIdx.x;
     const int fx = d->fx, fy = d->fy;
@@ -859,7 +859,11 @@ __device__ __forceinline__ void join_rows_body(const to_join_desc* __restrict__ 
     int64_t kx[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) kx[k] = k < K ? uniform64(pdep(k, d->xs_pos, S)) : 0;
-    const int tb = fx - 6;                   // tile bits of f
+    // a tile is 64 rows, or 8 passes of rb rows when that is fewer: a block's passes are
+    // sequential (each waits for its X loads), so wide rows get short tiles, more blocks
+    const int lr = min(min(fx, 6), 11 + (sizeof(T) == 4 ? 2 : 1) - fy);  // log2 rows per tile
+    const int64_t ROWS = 1ll << lr;
+    const int tb = fx - lr;                  // tile bits of f
     const int64_t nft = 1ll << tb;
     const int64_t ntiles = (int64_t)d->nbatch << tb;
     const int64_t t0 = ntiles * local / d->nblocks, t1 = ntiles * (local + 1) / d->nblocks;

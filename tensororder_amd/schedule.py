@@ -313,8 +313,9 @@ def choose_variant(g, nb, esize, launch_bound=False):
     # as whole rows (each thread keeps 16 bytes of y and their Y values in registers)
     if STREAM_ROWS and g["shy"] == 0 and g["shf"] == fy and 5 <= fy <= (10 if esize == 4 else 9) \
             and fx >= 6 and s <= 4 and not (launch_bound and fx + fy + math.log2(max(nb, 1)) < 20):
+        lr = min(fx, 6, 11 + (2 if esize == 4 else 1) - fy)  # log2 rows per tile (<= 8 passes)
         return dict(kind="stream", yc=16, nych=1, lds=0, stage=0, vec=False, rows=True,
-                    nblocks=int(min(nb << (fx - 6), MAX_STREAM_BLOCKS)))
+                    nblocks=int(min(nb << (fx - lr), MAX_STREAM_BLOCKS)))
     # 16-byte form: each thread owns 16 bytes of consecutive f (X's lowest bits are free)
     # and stores its 2^fy outputs per f as 16-byte chunks (all y of one f adjacent)
     lvf = 2 if esize == 4 else 1
