@@ -1652,7 +1652,7 @@ __global__ __launch_bounds__(kBlock, 1) void k_gemm128_db(const to_gemm_desc* __
 // Inputs below 2^-110 of a tensor's largest entry lose bits in the bf16 subnormal range
 // (the float path loses them below 2^-126).
 //
-// 128x128 tiles, 4 waves with a 64x64 quarter each, K slabs of 32 floats fetched 16 bytes
+// 128x128 tiles, 8 waves with a 64x32 part each, K slabs of 32 floats fetched 16 bytes
 // at a time two slabs ahead; each float is split once per block when its slab is written
 // to LDS, as three bf16 planes [plane][row][k] whose row pitch (40 bf16 = 80 bytes) keeps
 // the 16-byte fragment reads conflict-free; LDS double-buffered, one block per CU.  MFMA
@@ -1684,11 +1684,24 @@ __device__ __forceinline__ Mfma<float>::V mfma_bf16(U4 a, U4 b, Mfma<float>::V c
                                                   0, 0);
 }
 
+// Waves per block of the split product: 8, a 64x32 quarter-column strip each, two per SIMD
+// (200 VGPRs with the double accumulators).  With 4 waves of 64x64 (one per SIMD) a wave
+// waits on each fragment read with nothing else to issue; 8 waves measured 3-16 % faster
+// (more for the float-accumulator form, which then needed accumulation registers too).
+template <bool WIDE>
+constexpr int x6_waves() {
+    return 8;
+}
+
 template <bool SCALED, bool WIDE, bool TR>
 __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ descs, int ndesc) {
     constexpr int BM = 128, BN = 128, BK = 32;
+    constexpr int NW = x6_waves<WIDE>();             // waves per block
+    static_assert(NW == 4 || NW == 8, "4 or 8 waves");
+    constexpr int WN = NW / 2;                       // waves along n
+    constexpr int TJ = BN / WN / 16;                 // 16-column MFMA tiles per wave (4 or 2)
     constexpr int PITCH = BK + 8;                    // bf16 per LDS row
-    constexpr int NV = BM * (BK / 4) / kBlock;       // 16-byte vectors per thread and operand (4)
+    constexpr int NV = BM * (BK / 4) / (64 * NW);    // 16-byte vectors per thread and operand (4 or 2)
     constexpr int FLUSH = 4;                         // K steps per float partial sum (128 terms)
     typedef Mfma<float>::V V;
     typedef float L __attribute__((ext_vector_type(4)));
@@ -1697,7 +1710,7 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
     __shared__ __align__(16) __bf16 Bs[2][3][BN][PITCH];
     const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / WN, wn = wave % WN;
     const int64_t K = 1ll << d->klog;
     const int64_t tiles_m = 1ll << (d->mlog - 7), tiles_n = 1ll << (d->nlog - 7);
     const int64_t per_batch = tiles_m * tiles_n;
@@ -1709,7 +1722,7 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
     int vrow[NV], vk[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        vrow[i] = 32 * i + 8 * wave + ((lane >> 4) | (((lane >> 3) & 1) << 2));
+        vrow[i] = 8 * NW * i + 8 * wave + ((lane >> 4) | (((lane >> 3) & 1) << 2));
         vk[i] = (lane & 7) * 4;
     }
     // fragment offset of this lane inside a plane: row (lane & 15), k = 8 * (lane >> 4)
@@ -1731,20 +1744,20 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
             const OpScale x = op_scale(d->as, d->asb, batch), y = op_scale(d->bs, d->bsb, batch);
             sa = x.f; ea = x.e; sb = y.f; eb = y.e;
         }
-        V acc[4][4];
-        double wide[4][4][4];
+        V acc[4][TJ];
+        double wide[4][TJ][4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < TJ; ++j)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     acc[i][j][v] = 0.0f;
                     wide[i][j][v] = 0.0;
                 }
         // Software pipeline over K slabs of 32: while K step s multiplies LDS buffer
-        // s & 1, each of the thread's 8 fetched vectors of slab s + 1 (4 of A, 4 of B) is
-        // split into the other buffer between two products, and its register is refilled
+        // s & 1, each of the thread's 2 NV fetched vectors of slab s + 1 (NV of A, NV of B)
+        // is split into the other buffer between two products, and its register is refilled
         // at once from slab s + 2.  A vector is thus split at the same point of every step,
         // one step after its fetch, and the splitting (VALU) runs beside the MFMAs.
         // Fetches past the end re-read the last slab (in bounds; what they stage is never
@@ -1766,12 +1779,13 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
             *reinterpret_cast<U2*>(&P[1][vrow[i]][vk[i]]) = U2{m0, m1};
             *reinterpret_cast<U2*>(&P[2][vrow[i]][vk[i]]) = U2{l0, l1};
         };
-        // stage two vectors of slab s + 1 into `nbuf` and fetch them from slab s + 2 (at k2)
-        auto pipe = [&](int v, int nbuf, int64_t k2) {
-            stage_vec(v, nbuf);
-            stage_vec(v + 1, nbuf);
-            fetch_vec(v, k2);
-            fetch_vec(v + 1, k2);
+        // stage the q-th quarter of the vectors (NV / 2) of slab s + 1 into `nbuf` and fetch
+        // them from slab s + 2 (at k2)
+        auto pipe = [&](int q, int nbuf, int64_t k2) {
+#pragma unroll
+            for (int u = 0; u < NV / 2; ++u) stage_vec(q * (NV / 2) + u, nbuf);
+#pragma unroll
+            for (int u = 0; u < NV / 2; ++u) fetch_vec(q * (NV / 2) + u, k2);
         };
         // The six products of LDS buffer `buf`, the large one last (a float accumulator
         // that takes the small terms first rounds less): (l,h) (m,h) (m,m) (h,m) (h,l) (h,h).
@@ -1789,35 +1803,36 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
             };
             auto read_b = [&](U4 (&f)[4], int plane) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    f[j] = *reinterpret_cast<const U4*>(&Bs[buf][plane][wn * 64 + j * 16][0] + frag);
+                for (int j = 0; j < TJ; ++j)
+                    f[j] = *reinterpret_cast<const U4*>(&Bs[buf][plane][wn * 16 * TJ + j * 16][0] + frag);
             };
             auto product = [&](const U4 (&a)[4], const U4 (&b)[4]) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
+                    for (int j = 0; j < TJ; ++j)
                         acc[i][j] = TR ? mfma_bf16(b[j], a[i], acc[i][j]) : mfma_bf16(a[i], b[j], acc[i][j]);
             };
             read_b(fb, 0);   // B h
             read_a(fa, 2);   // A l
             read_a(fn, 1);   // A m (next)
-            // Interleave the splitting of two vectors (about 44 VALU) with the 16 MFMAs of
-            // one product: while an MFMA runs (16 cycles), the wave issues the next VALU
-            // instructions.
+            // Interleave the splitting of NV / 2 vectors (about 22 VALU each) with the
+            // 4 TJ MFMAs of one product: while an MFMA runs (16 cycles), the wave issues
+            // the next VALU instructions.
             auto interleave = [&]() {
+                constexpr int QN = NW == 4 ? 11 : 6;
 #pragma unroll
-                for (int q = 0; q < 11; ++q) {
+                for (int q = 0; q < QN; ++q) {
                     __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
                     __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // four VALU
                 }
-                __builtin_amdgcn_sched_group_barrier(0x008, 5, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 4 * TJ - QN, 0);
             };
             if (fresh) {     // the first product of a partial sum starts from zero
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
+                    for (int j = 0; j < TJ; ++j)
                         acc[i][j] = TR ? mfma_bf16(fb[j], fa[i], V{0.0f, 0.0f, 0.0f, 0.0f})
                                        : mfma_bf16(fa[i], fb[j], V{0.0f, 0.0f, 0.0f, 0.0f});
             } else {
@@ -1828,17 +1843,17 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
             __builtin_amdgcn_sched_barrier(0);
             read_b(fa, 1);   // B m (next; A l is done)
             product(fn, fb);                       // (m,h)
-            pipe(2, nbuf, k2);
+            pipe(1, nbuf, k2);
             interleave();
             __builtin_amdgcn_sched_barrier(0);
             read_a(fb, 0);   // A h (next; B h is done)
             product(fn, fa);                       // (m,m)
-            pipe(4, nbuf, k2);
+            pipe(2, nbuf, k2);
             interleave();
             __builtin_amdgcn_sched_barrier(0);
             read_b(fn, 2);   // B l (next; A m is done)
             product(fb, fa);                       // (h,m)
-            pipe(6, nbuf, k2);
+            pipe(3, nbuf, k2);
             interleave();
             __builtin_amdgcn_sched_barrier(0);
             read_b(fa, 0);   // B h again (next; B m is done)
@@ -1869,7 +1884,7 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
 #pragma unroll
-                        for (int j = 0; j < 4; ++j)
+                        for (int j = 0; j < TJ; ++j)
 #pragma unroll
                             for (int v = 0; v < 4; ++v) wide[i][j][v] += (double)acc[i][j][v];
                 }
@@ -1883,14 +1898,14 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < TJ; ++j) {
                 float out[4];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     out[v] = (WIDE ? (float)wide[i][j][v] : acc[i][j][v]) * sab;
                     if (SCALED) mx = fmaxf(mx, fabsf(out[v]));
                 }
-                const int64_t mb = tm * BM + wm * 64 + i * 16, nb = tn * BN + wn * 64 + j * 16;
+                const int64_t mb = tm * BM + wm * 64 + i * 16, nb = tn * BN + wn * 16 * TJ + j * 16;
                 if (vec_out) {
                     const int64_t m = TR ? mb + (lane & 15) : mb + Mfma<float>::row(lane, 0);
                     const int64_t n = TR ? nb + Mfma<float>::row(lane, 0) : nb + (lane & 15);
@@ -1910,10 +1925,9 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
     if (SCALED) publish_max_on_change(d->cs, mx_batch, -1, mx, lane);
 }
 
-// one block per CU (the double accumulators need the register file of one wave per SIMD),
-// LDS double-buffered
+// one block per CU, LDS double-buffered
 template <bool SCALED, bool WIDE, bool TR>
-__global__ __launch_bounds__(kBlock, 1) void k_gemm_x6(const to_gemm_desc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(64 * x6_waves<WIDE>(), 1) void k_gemm_x6(const to_gemm_desc* __restrict__ descs, int ndesc) {
     gemm_x6_body<SCALED, WIDE, TR>(descs, ndesc);
 }
 
@@ -2037,16 +2051,16 @@ int launch_one(const to_launch& L, hipStream_t st) {
                     const to_gemm_desc* gd = reinterpret_cast<const to_gemm_desc*>(L.descs);
                     switch (L.variant) {
                         case 25:
-                            hipLaunchKernelGGL((k_gemm_x6<SCALED, false, false>), grid, block, 0, st, gd, L.count);
+                            hipLaunchKernelGGL((k_gemm_x6<SCALED, false, false>), grid, dim3(64 * x6_waves<false>()), 0, st, gd, L.count);
                             break;
                         case 27:
-                            hipLaunchKernelGGL((k_gemm_x6<SCALED, true, false>), grid, block, 0, st, gd, L.count);
+                            hipLaunchKernelGGL((k_gemm_x6<SCALED, true, false>), grid, dim3(64 * x6_waves<true>()), 0, st, gd, L.count);
                             break;
                         case 29:
-                            hipLaunchKernelGGL((k_gemm_x6<SCALED, false, true>), grid, block, 0, st, gd, L.count);
+                            hipLaunchKernelGGL((k_gemm_x6<SCALED, false, true>), grid, dim3(64 * x6_waves<false>()), 0, st, gd, L.count);
                             break;
                         case 31:
-                            hipLaunchKernelGGL((k_gemm_x6<SCALED, true, true>), grid, block, 0, st, gd, L.count);
+                            hipLaunchKernelGGL((k_gemm_x6<SCALED, true, true>), grid, dim3(64 * x6_waves<true>()), 0, st, gd, L.count);
                             break;
                         default:  // the split product runs at one block per CU (bit 3)
                             set_err("gemm: unknown bf16 split-product variant");
