@@ -23,6 +23,9 @@ ALIGN = schedule.ALIGN
 KIND_ORDER = schedule.KIND_ORDER
 # one launch for all streamed joins of a step (TENSORORDER_BATCH_MERGE_STREAM=0: one per form)
 MERGE_STREAM_FORMS = os.environ.get("TENSORORDER_BATCH_MERGE_STREAM", "1") != "0"
+# steps by tree level (every join of level k of every instance in step k) rather than by
+# join index (the k-th join of every instance); TENSORORDER_BATCH_LEVELS=0 for the latter
+LEVEL_STEPS = os.environ.get("TENSORORDER_BATCH_LEVELS", "1") != "0"
 
 
 class BatchProgram:
@@ -49,7 +52,7 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
                 g = schedule.join_geometry(nodes[n.left].rank, nodes[n.right].rank, n.lmap, n.rmap)
                 g["var"] = schedule.choose_variant(g, 1, esize, launch_bound=True)
                 n.geom = g
-        peak, scratch = schedule.plan_memory(nodes, tables, 1, esize)
+        peak, scratch = schedule.plan_memory(nodes, tables, 1, esize, "level" if LEVEL_STEPS else "post")
         per.append(dict(nodes=nodes, tables=tables, base=arena_top, scratch=scratch, peak=peak))
         arena_top += (max(peak, ALIGN) + ALIGN - 1) // ALIGN * ALIGN
 
@@ -113,15 +116,20 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
                 n.ptr = abase + inst["base"] + n.buf[0]
             n.bstride = 0
 
-    # group every instance's k-th join into step k
+    # group every instance's joins of level k (or its k-th join) into step k
     steps = {}
     roots = {}
     for i, inst in enumerate(per):
         nodes = inst["nodes"]
         joins = [n for n in nodes if not n.is_leaf]
+        if LEVEL_STEPS:
+            step_of = {j: k for k, grp in enumerate(schedule.join_levels(nodes, False)) for j in grp}
+        else:
+            step_of = {n.idx: k for k, n in enumerate(joins)}
         scratch = {k: {kk: (inst["base"] + blk[0], blk[1]) for kk, blk in v.items()}
                    for k, v in inst["scratch"].items()}
-        for k, n in enumerate(joins):
+        for n in joins:
+            k = step_of[n.idx]
             for kind, op, variant, lds, st, nblk in schedule.join_records(
                     n, nodes, 1, esize, scaled, abase, scratch.get(n.idx, {}), recp):
                 # the streamed joins of one step share one launch whatever their form
@@ -138,7 +146,7 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
                 steps.setdefault(k, {}).setdefault(key, []).append((st, nblk, lds, kind, i, work))
         root = nodes[-1]
         rd = _lib.RootDesc(root=root.ptr, rec=recp(root), acc=prog.accum.data_ptr() + i * acc_size)
-        roots.setdefault(len(joins), []).append(rd)
+        roots.setdefault(max(step_of.values()) + 1 if step_of else 0, []).append(rd)
 
     blob = bytearray()
 
