@@ -26,6 +26,9 @@ MERGE_STREAM_FORMS = os.environ.get("TENSORORDER_BATCH_MERGE_STREAM", "1") != "0
 # steps by tree level (every join of level k of every instance in step k) rather than by
 # join index (the k-th join of every instance); TENSORORDER_BATCH_LEVELS=0 for the latter
 LEVEL_STEPS = os.environ.get("TENSORORDER_BATCH_LEVELS", "1") != "0"
+# kernel forms chosen for fewer launches (choose_variant launch_bound); 0 for the per-join
+# choice of a single contraction
+LAUNCH_BOUND_FORMS = os.environ.get("TENSORORDER_BATCH_LAUNCH_BOUND", "1") != "0"
 
 
 class BatchProgram:
@@ -50,7 +53,7 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
         for n in nodes:
             if not n.is_leaf:
                 g = schedule.join_geometry(nodes[n.left].rank, nodes[n.right].rank, n.lmap, n.rmap)
-                g["var"] = schedule.choose_variant(g, 1, esize, launch_bound=True)
+                g["var"] = schedule.choose_variant(g, 1, esize, launch_bound=LAUNCH_BOUND_FORMS)
                 n.geom = g
         peak, scratch = schedule.plan_memory(nodes, tables, 1, esize, "level" if LEVEL_STEPS else "post")
         per.append(dict(nodes=nodes, tables=tables, base=arena_top, scratch=scratch, peak=peak))
