@@ -188,11 +188,17 @@ typedef struct to_launch {
     int32_t count;  /* descriptors */
     int32_t grid;   /* blocks (256 threads each) */
     int32_t lds;    /* dynamic LDS bytes */
-    int32_t pad;
+    int32_t flags;  /* TO_LAUNCH_* (graph capture only; to_run runs records in order) */
     uint64_t descs; /* device pointer to the descriptor array */
     uint64_t aux0, aux1, aux2, aux3;
     int64_t iaux0, iaux1;
 } to_launch;
+
+/* to_launch.flags, read by to_graph_create: the record runs on a second captured stream
+ * (SIDE), and/or the side stream is joined into the main one and forked from it again
+ * before the record (JOIN), so SIDE records between two JOINs run beside the main ones. */
+#define TO_LAUNCH_SIDE 0x1
+#define TO_LAUNCH_JOIN 0x2
 
 /* Library ABI version (bumped when a struct layout changes). */
 int to_version(void);
@@ -220,7 +226,9 @@ int to_run(const to_launch* launches, int count, int entry_type, void* stream);
  * waits for completion and stores each record's device time in ms[i] (count floats). */
 int to_run_timed(const to_launch* launches, int count, int entry_type, void* stream, float* ms);
 
-/* Capture the records into a HIP graph once; replay it with to_graph_launch. */
+/* Capture the records into a HIP graph once; replay it with to_graph_launch.  Records
+ * flagged TO_LAUNCH_SIDE are captured on a second stream (created and destroyed here,
+ * default priority) between TO_LAUNCH_JOIN points; the graph ends joined. */
 int to_graph_create(const to_launch* launches, int count, int entry_type, void* stream,
                     void** graph_exec);
 int to_graph_launch(void* graph_exec, void* stream);

@@ -29,8 +29,10 @@ TO_F64, TO_F32S = 0, 1
 TO_STREAM_VEC = 0x100  # stream variant bit: the 16-byte form
 TO_STREAM_ROWS = 0x200  # stream variant: the row form
 TO_STREAM_ANY = 0x400  # stream launch variant: mixed joins, each by its descriptor's kvar
+TO_LAUNCH_SIDE = 0x1  # Launch.flags: captured on the side stream
+TO_LAUNCH_JOIN = 0x2  # Launch.flags: join the side stream into the main one and fork it again first
 MAX_BITS = 32
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 c_i8x32 = ctypes.c_int8 * MAX_BITS
 
@@ -106,7 +108,7 @@ class RootDesc(ctypes.Structure):
 class Launch(ctypes.Structure):
     _fields_ = [
         ("op", ctypes.c_int32), ("variant", ctypes.c_int32), ("count", ctypes.c_int32),
-        ("grid", ctypes.c_int32), ("lds", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("grid", ctypes.c_int32), ("lds", ctypes.c_int32), ("flags", ctypes.c_int32),
         ("descs", ctypes.c_uint64),
         ("aux0", ctypes.c_uint64), ("aux1", ctypes.c_uint64),
         ("aux2", ctypes.c_uint64), ("aux3", ctypes.c_uint64),

@@ -29,6 +29,10 @@ LEVEL_STEPS = os.environ.get("TENSORORDER_BATCH_LEVELS", "1") != "0"
 # kernel forms chosen for fewer launches (choose_variant launch_bound); 0 for the per-join
 # choice of a single contraction
 LAUNCH_BOUND_FORMS = os.environ.get("TENSORORDER_BATCH_LAUNCH_BOUND", "1") != "0"
+# in the batch graph, a step's streamed joins run on a side branch beside its operand
+# permutes, products and split-K joins (joins of one level are independent; the branches
+# join at every step boundary); TENSORORDER_BATCH_BRANCHES=0 keeps one chain
+BRANCHES = os.environ.get("TENSORORDER_BATCH_BRANCHES", "1") != "0"
 
 
 class BatchProgram:
@@ -172,6 +176,7 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
     nsteps = max(list(steps) + list(roots)) + 1 if (steps or roots) else 0
     launches_per_kind = {}
     for k in range(nsteps):
+        step_start = len(pending)
         for key in sorted(steps.get(k, {})):
             group = steps[k][key]
             structs, block0, lds, wb, wf = [], 0, 0, 0, 0
@@ -184,14 +189,18 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
                 wf += work[1]
             off = put(structs)
             kind = group[0][3]
-            pending.append((Launch(op=key[1], variant=key[2], count=len(structs), grid=block0, lds=lds), off,
-                            (kind, k), (wb, wf)))
+            side = BRANCHES and key[1] == _lib.TO_OP_JOIN_STREAM
+            pending.append((Launch(op=key[1], variant=key[2], count=len(structs), grid=block0, lds=lds,
+                                   flags=_lib.TO_LAUNCH_SIDE if side else 0), off, (kind, k), (wb, wf)))
             launches_per_kind[kind] = launches_per_kind.get(kind, 0) + 1
         if k in roots:
             rs = roots[k]
             off = put(rs)
             pending.append((Launch(op=_lib.TO_OP_ROOT_STORE, count=len(rs), grid=max(1, -(-len(rs) // 256))),
                             off, ("root", k), (0, 0)))
+        if BRANCHES and len(pending) > step_start:
+            # the step starts from everything of the previous one, on both branches
+            pending[step_start][0].flags |= _lib.TO_LAUNCH_JOIN
     prog.blob_host = np.frombuffer(bytes(blob) or b"\0", dtype=np.uint8).copy()
     prog.blob = device_alloc(prog.blob_host.nbytes)
     bbase = prog.blob.data_ptr()

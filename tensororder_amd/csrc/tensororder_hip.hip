@@ -24,7 +24,7 @@
 
 #include "tensororder_hip.h"
 
-#define TO_ABI_VERSION 4
+#define TO_ABI_VERSION 5
 
 namespace {
 
@@ -2216,11 +2216,67 @@ int to_graph_create(const to_launch* launches, int count, int entry_type, void* 
         return TO_ERR_INVALID;
     }
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-    const int rc = run_records(launches, count, entry_type, st);
+    if (!launches || count < 0) {
+        set_err("null launch list");
+        return TO_ERR_INVALID;
+    }
+    bool branched = false;
+    for (int i = 0; i < count; ++i) branched |= (launches[i].flags & TO_LAUNCH_SIDE) != 0;
+    // the side branch: a second stream that joins the capture through events
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    auto release = [&]() {
+        if (fork) hipEventDestroy(fork);
+        if (join) hipEventDestroy(join);
+        if (side) hipStreamDestroy(side);
+    };
+    hipError_t e = hipSuccess;
+    if (branched) {
+        e = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&join, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            release();
+            return hip_fail(e, "side stream");
+        }
+    }
+    e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(e, "hipStreamBeginCapture");
+    }
+    auto fork_side = [&]() {  // the side stream continues from the main stream's point
+        hipError_t r = hipEventRecord(fork, st);
+        return r == hipSuccess ? hipStreamWaitEvent(side, fork, 0) : r;
+    };
+    auto join_side = [&]() {  // the main stream waits for everything on the side stream
+        hipError_t r = hipEventRecord(join, side);
+        return r == hipSuccess ? hipStreamWaitEvent(st, join, 0) : r;
+    };
+    int rc = TO_OK;
+    if (branched) {
+        const hipError_t r = fork_side();
+        if (r != hipSuccess) rc = hip_fail(r, "fork");
+    }
+    for (int i = 0; i < count && rc == TO_OK; ++i) {
+        const to_launch& L = launches[i];
+        if (branched && (L.flags & TO_LAUNCH_JOIN)) {
+            hipError_t r = join_side();
+            if (r == hipSuccess) r = fork_side();
+            if (r != hipSuccess) {
+                rc = hip_fail(r, "join");
+                break;
+            }
+        }
+        rc = run_records(&L, 1, entry_type, branched && (L.flags & TO_LAUNCH_SIDE) ? side : st);
+    }
+    if (branched) {
+        const hipError_t r = join_side();  // the capture must end joined
+        if (rc == TO_OK && r != hipSuccess) rc = hip_fail(r, "join");
+    }
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(st, &graph);
+    release();
     if (rc != TO_OK) {
         if (graph) hipGraphDestroy(graph);
         return rc;

@@ -161,6 +161,18 @@ def test_batch_of_uneven_instances_emulated(et, tol):
     assert prog.stats["steps"] < prog.stats["joins"]
     for g, w in zip(got, wants):
         assert g == pytest.approx(w, rel=tol)
+    # graph branches: only streamed joins go to the side stream, and every step starts
+    # with a join of the two branches (so a step sees all of the previous one)
+    first = {}
+    for i, (L, (kind, k)) in enumerate(zip(prog.launches, prog.info)):
+        if L.flags & _lib.TO_LAUNCH_SIDE:
+            assert L.op == _lib.TO_OP_JOIN_STREAM
+        if k >= 0:
+            first.setdefault(k, i)
+    assert any(L.flags & _lib.TO_LAUNCH_SIDE for L in prog.launches)
+    for k, i in first.items():
+        assert prog.launches[i].flags & _lib.TO_LAUNCH_JOIN, k
+    assert sum(1 for L in prog.launches if L.flags & _lib.TO_LAUNCH_JOIN) == len(first)
 
 
 def _gemm_descs(prog):
