@@ -1,12 +1,13 @@
 """Many independent instances in one schedule (a batch of unsliced plans of uneven size).
 
-Each instance keeps its own compiled post-order join list and its own arena region.  Step
-k of the batch runs the k-th join of every instance that has one, as grouped launches (one
-per kernel variant) whose blocks locate their join by binary search over ``block0`` --
-so a batch of N instances needs max_i(#joins_i) steps rather than sum_i(#joins_i)
-launches, and independent small joins of different instances share the machine.  Right
-after its last join an instance's root is stored into its own accumulator together with
-the device clock, which gives per-instance completion times within the batch.
+Each instance keeps its own compiled join list and its own arena region (placed for level
+order).  Step k of the batch runs every join of tree level k of every instance (a join's
+level is one more than its children's; joins of one level are independent), as grouped
+launches (one per kernel variant) whose blocks locate their join by binary search over
+``block0`` -- so a batch of N instances needs max_i(height_i) steps rather than
+sum_i(#joins_i) launches, and independent small joins share the machine.  Right after its
+last level an instance's root is stored into its own accumulator together with the device
+clock, which gives per-instance completion times within the batch.
 """
 
 import ctypes
