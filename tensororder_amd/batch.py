@@ -34,6 +34,9 @@ LAUNCH_BOUND_FORMS = os.environ.get("TENSORORDER_BATCH_LAUNCH_BOUND", "1") != "0
 # permutes, products and split-K joins (joins of one level are independent; the branches
 # join at every step boundary); TENSORORDER_BATCH_BRANCHES=0 keeps one chain
 BRANCHES = os.environ.get("TENSORORDER_BATCH_BRANCHES", "1") != "0"
+# split-K joins (their operand permutes, partial sums and finishing) on the side branch too,
+# which balances the branches; TENSORORDER_BATCH_SIDE_SPLITK=0 keeps them on the main one
+SIDE_SPLITK = os.environ.get("TENSORORDER_BATCH_SIDE_SPLITK", "1") != "0"
 
 
 class BatchProgram:
@@ -136,15 +139,17 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
             step_of = {n.idx: k for k, n in enumerate(joins)}
         scratch = {k: {kk: (inst["base"] + blk[0], blk[1]) for kk, blk in v.items()}
                    for k, v in inst["scratch"].items()}
+        side_kinds = ("stream", "splitk") if SIDE_SPLITK else ("stream",)
         for n in joins:
             k = step_of[n.idx]
+            side = BRANCHES and n.geom["var"]["kind"] in side_kinds
             for kind, op, variant, lds, st, nblk in schedule.join_records(
                     n, nodes, 1, esize, scaled, abase, scratch.get(n.idx, {}), recp):
                 # the streamed joins of one step share one launch whatever their form
                 # (each descriptor names its own); the step is bound by launch count
                 if op == _lib.TO_OP_JOIN_STREAM and MERGE_STREAM_FORMS:
                     variant = _lib.TO_STREAM_ANY
-                key = (KIND_ORDER[kind], op, variant)
+                key = (side, KIND_ORDER[kind], op, variant)
                 if kind == "permute":
                     work = (2 * esize * (int(st.nbatch) << int(st.rank)), 0)
                 elif kind in ("stream", "splitk", "gemm"):
@@ -190,9 +195,8 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
                 wf += work[1]
             off = put(structs)
             kind = group[0][3]
-            side = BRANCHES and key[1] == _lib.TO_OP_JOIN_STREAM
-            pending.append((Launch(op=key[1], variant=key[2], count=len(structs), grid=block0, lds=lds,
-                                   flags=_lib.TO_LAUNCH_SIDE if side else 0), off, (kind, k), (wb, wf)))
+            pending.append((Launch(op=key[2], variant=key[3], count=len(structs), grid=block0, lds=lds,
+                                   flags=_lib.TO_LAUNCH_SIDE if key[0] else 0), off, (kind, k), (wb, wf)))
             launches_per_kind[kind] = launches_per_kind.get(kind, 0) + 1
         if k in roots:
             rs = roots[k]

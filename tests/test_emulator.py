@@ -161,12 +161,16 @@ def test_batch_of_uneven_instances_emulated(et, tol):
     assert prog.stats["steps"] < prog.stats["joins"]
     for g, w in zip(got, wants):
         assert g == pytest.approx(w, rel=tol)
-    # graph branches: only streamed joins go to the side stream, and every step starts
-    # with a join of the two branches (so a step sees all of the previous one)
+    # graph branches: streamed and split-K joins (with their operand permutes) go to the
+    # side stream, products stay on the main one, and every step starts with a join of the
+    # two branches (so a step sees all of the previous one)
     first = {}
+    side_ops = {_lib.TO_OP_JOIN_STREAM, _lib.TO_OP_JOIN_SPLITK, _lib.TO_OP_JOIN_FINISH, _lib.TO_OP_PERMUTE}
     for i, (L, (kind, k)) in enumerate(zip(prog.launches, prog.info)):
         if L.flags & _lib.TO_LAUNCH_SIDE:
-            assert L.op == _lib.TO_OP_JOIN_STREAM
+            assert L.op in side_ops
+        if L.op == _lib.TO_OP_JOIN_STREAM:
+            assert L.flags & _lib.TO_LAUNCH_SIDE
         if k >= 0:
             first.setdefault(k, i)
     assert any(L.flags & _lib.TO_LAUNCH_SIDE for L in prog.launches)
