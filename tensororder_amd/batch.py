@@ -11,9 +11,6 @@ clock, which gives per-instance completion times within the batch.
 """
 
 import ctypes
-import math
-
-import os
 
 import numpy as np
 
@@ -22,21 +19,6 @@ from ._lib import Launch
 
 ALIGN = schedule.ALIGN
 KIND_ORDER = schedule.KIND_ORDER
-# one launch for all streamed joins of a step (TENSORORDER_BATCH_MERGE_STREAM=0: one per form)
-MERGE_STREAM_FORMS = os.environ.get("TENSORORDER_BATCH_MERGE_STREAM", "1") != "0"
-# steps by tree level (every join of level k of every instance in step k) rather than by
-# join index (the k-th join of every instance); TENSORORDER_BATCH_LEVELS=0 for the latter
-LEVEL_STEPS = os.environ.get("TENSORORDER_BATCH_LEVELS", "1") != "0"
-# kernel forms chosen for fewer launches (choose_variant launch_bound); 0 for the per-join
-# choice of a single contraction
-LAUNCH_BOUND_FORMS = os.environ.get("TENSORORDER_BATCH_LAUNCH_BOUND", "1") != "0"
-# in the batch graph, a step's streamed joins run on a side branch beside its operand
-# permutes, products and split-K joins (joins of one level are independent; the branches
-# join at every step boundary); TENSORORDER_BATCH_BRANCHES=0 keeps one chain
-BRANCHES = os.environ.get("TENSORORDER_BATCH_BRANCHES", "1") != "0"
-# split-K joins (their operand permutes, partial sums and finishing) on the side branch too,
-# which balances the branches; TENSORORDER_BATCH_SIDE_SPLITK=0 keeps them on the main one
-SIDE_SPLITK = os.environ.get("TENSORORDER_BATCH_SIDE_SPLITK", "1") != "0"
 
 
 class BatchProgram:
@@ -61,9 +43,10 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
         for n in nodes:
             if not n.is_leaf:
                 g = schedule.join_geometry(nodes[n.left].rank, nodes[n.right].rank, n.lmap, n.rmap)
-                g["var"] = schedule.choose_variant(g, 1, esize, launch_bound=LAUNCH_BOUND_FORMS)
+                # kernel forms chosen for fewer launches: the batch is bound by launch count
+                g["var"] = schedule.choose_variant(g, 1, esize, launch_bound=True)
                 n.geom = g
-        peak, scratch = schedule.plan_memory(nodes, tables, 1, esize, "level" if LEVEL_STEPS else "post")
+        peak, scratch = schedule.plan_memory(nodes, tables, 1, esize, "level")
         per.append(dict(nodes=nodes, tables=tables, base=arena_top, scratch=scratch, peak=peak))
         arena_top += (max(peak, ALIGN) + ALIGN - 1) // ALIGN * ALIGN
 
@@ -72,18 +55,20 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
     # leaf tables
     chunks, off = [], 0
     for inst in per:
-        inst["table_off"], inst["exp0"] = {}, {}
+        inst["table_off"], inst["exp0"], factor_of = {}, {}, {}
         for n in inst["nodes"]:
             if n.is_leaf and n.tensor not in inst["table_off"]:
                 rows = inst["tables"][n.tensor].rows
                 if scaled:
                     rows, F, e = schedule.leaf_encoding(rows)
                     inst["exp0"][n.tensor] = e
-                    inst.setdefault("factors", []).append(F)
+                    factor_of[n.tensor] = F
                 arr = np.ascontiguousarray(rows[:1], dtype=fdtype).ravel()
                 inst["table_off"][n.tensor] = off
                 chunks.append((off, arr))
                 off += (arr.size * esize + ALIGN - 1) // ALIGN * ALIGN // esize
+        # one factor per leaf node (a tensor at two leaves enters the count twice)
+        inst["factors"] = [factor_of[n.tensor] for n in inst["nodes"] if n.is_leaf] if scaled else []
     table_host = np.zeros(max(off, 1), dtype=fdtype)
     for o, arr in chunks:
         table_host[o:o + arr.size] = arr
@@ -127,27 +112,27 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
                 n.ptr = abase + inst["base"] + n.buf[0]
             n.bstride = 0
 
-    # group every instance's joins of level k (or its k-th join) into step k
+    # group every instance's joins of tree level k into step k.  In the batch graph a step's
+    # streamed and split-K joins (with their operand permutes and finishing) run on a side
+    # branch beside its permute -> product chain (joins of one level are independent), and
+    # the branches join at every step boundary.
     steps = {}
     roots = {}
     for i, inst in enumerate(per):
         nodes = inst["nodes"]
         joins = [n for n in nodes if not n.is_leaf]
-        if LEVEL_STEPS:
-            step_of = {j: k for k, grp in enumerate(schedule.join_levels(nodes, False)) for j in grp}
-        else:
-            step_of = {n.idx: k for k, n in enumerate(joins)}
+        step_of = {j: k for k, grp in enumerate(schedule.join_levels(nodes, False)) for j in grp}
         scratch = {k: {kk: (inst["base"] + blk[0], blk[1]) for kk, blk in v.items()}
                    for k, v in inst["scratch"].items()}
-        side_kinds = ("stream", "splitk") if SIDE_SPLITK else ("stream",)
         for n in joins:
             k = step_of[n.idx]
-            side = BRANCHES and n.geom["var"]["kind"] in side_kinds
+            side = n.geom["var"]["kind"] in ("stream", "splitk")
             for kind, op, variant, lds, st, nblk in schedule.join_records(
                     n, nodes, 1, esize, scaled, abase, scratch.get(n.idx, {}), recp):
                 # the streamed joins of one step share one launch whatever their form
                 # (each descriptor names its own); the step is bound by launch count
-                if op == _lib.TO_OP_JOIN_STREAM and MERGE_STREAM_FORMS:
+                if op == _lib.TO_OP_JOIN_STREAM:
+                    schedule.check_stream_form(st.kvar)
                     variant = _lib.TO_STREAM_ANY
                 key = (side, KIND_ORDER[kind], op, variant)
                 if kind == "permute":
@@ -203,7 +188,7 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
             off = put(rs)
             pending.append((Launch(op=_lib.TO_OP_ROOT_STORE, count=len(rs), grid=max(1, -(-len(rs) // 256))),
                             off, ("root", k), (0, 0)))
-        if BRANCHES and len(pending) > step_start:
+        if len(pending) > step_start:
             # the step starts from everything of the previous one, on both branches
             pending[step_start][0].flags |= _lib.TO_LAUNCH_JOIN
     prog.blob_host = np.frombuffer(bytes(blob) or b"\0", dtype=np.uint8).copy()

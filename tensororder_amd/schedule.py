@@ -267,6 +267,18 @@ def stream_variant(v, g):
     return v["yc"] | (_lib.TO_STREAM_VEC if v["vec"] else 0)
 
 
+# the forms a descriptor of a mixed streamed-join launch (TO_STREAM_ANY) may name
+STREAM_FORMS = frozenset([yc for yc in (1, 2, 4, 8, 16)] + [_lib.TO_STREAM_VEC | yc for yc in (1, 2, 4, 8, 16)]
+                         + [_lib.TO_STREAM_ROWS])
+
+
+def check_stream_form(kvar):
+    """Reject a streamed-join descriptor whose form a TO_STREAM_ANY launch cannot run
+    (the kernel would leave that join's output unwritten)."""
+    if int(kvar) not in STREAM_FORMS:
+        raise ValueError("streamed join: unknown form %#x" % int(kvar))
+
+
 def choose_variant(g, nb, esize, launch_bound=False):
     """Kernel variant and launch shape for a join with ``nb`` batch items.  With
     ``launch_bound`` (the many-instance batch, whose steps group the joins of all instances
@@ -473,9 +485,10 @@ def leaf_encoding(rows):
     A table whose nonzero magnitudes take at most two values -- a variable's weight pair
     (w(not x), w(x)), or 0/1 clause entries -- is stored exactly: the two magnitudes become
     the integers a, b < 2^24 of the best rational approximation b/a of their ratio, and F
-    carries the rest (relative error of the ratio ~1e-13).  Every leaf enters each slice's
-    contraction exactly once, so the whole count scales by the product of the leaves' F,
-    which the host divides out in FP64.  Other tables are rounded to float32 (F = 1)."""
+    carries the rest (relative error of the ratio ~1e-13).  Each leaf node of the tree enters
+    each slice's contraction once, so the whole count scales by the product of F over the
+    leaf nodes (a tensor at two leaves counts twice), which the host multiplies back in
+    FP64.  Other tables are rounded to float32 (F = 1)."""
     rows = np.asarray(rows, dtype=np.float64)
     mags = np.unique(np.abs(rows[rows != 0]))
     F = 1.0
@@ -758,15 +771,14 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
     # ---- leaf tables (host) -> one device buffer; the scaled executor's leaf factors
     # multiply the count (prog.leaf_factor, mantissa and exponent)
     leaf_ids = sorted({n.tensor for n in nodes if n.is_leaf})
-    table_off, chunks, exp0 = {}, [], {}
-    factors = []
+    table_off, chunks, exp0, factor_of = {}, [], {}, {}
     off = 0
     for t in leaf_ids:
         rows = tables[t].rows
         if scaled:
             rows, F, e = leaf_encoding(rows)
             exp0[t] = e
-            factors.append(F)
+            factor_of[t] = F
         arr = np.ascontiguousarray(rows, dtype=fdtype).ravel()
         table_off[t] = off
         chunks.append(arr)
@@ -774,7 +786,11 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
     table_host = np.zeros(max(off, 1), dtype=fdtype)
     for t, arr in zip(leaf_ids, chunks):
         table_host[table_off[t]:table_off[t] + arr.size] = arr
-    prog.leaf_factor = factor_product(factors)
+    # one factor per leaf *occurrence*: a tree may hold one tensor at several leaves (the
+    # line planner joins a rank-0 tensor at its first bag and again at the top,
+    # line_graph_method.py:16-35 and contraction_tree.pyx:302-310), and each occurrence
+    # multiplies the count by that tensor's values
+    prog.leaf_factor = factor_product([factor_of[n.tensor] for n in nodes if n.is_leaf] if scaled else [])
     coef_host = np.zeros(max(1, len(leaf_ids) * max(1, num_groups)), dtype=np.int64)
     coef_off = {}
     for i, t in enumerate(leaf_ids):

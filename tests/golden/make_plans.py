@@ -54,7 +54,7 @@ SPECS = {
     # factor-Flow plan of max rank 60 (treewidth 53): 12 slices leave rank 48, far beyond
     # any memory.  70 variables at the same clause ratio gives rank 38 -> 25 with 12 slices.
     "cnf70_factor_s12": (lambda: random_3cnf(70, 140, 0, True), "cachet", "factor", 120, {"slices": 12},
-                         ["prefix:2"]),
+                         ["prefix:32"]),
 }
 
 

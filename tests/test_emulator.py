@@ -221,3 +221,86 @@ def test_gemm_descriptors_fit_their_tiles(et):
                 assert f32 and variant & 1 and variant & 8 and gd.nlog >= 7
                 assert gd.klog >= (7 if variant & 2 else 5)  # wide: whole partial sums of 4 slabs
     assert seen > 0
+
+
+# DIMACS inputs that declare variables no clause uses.  The line planner matches a rank-0
+# variable tensor at its first bag and joins it again at the top (the reference's
+# line_graph_method.py:16-35 with contraction_tree.pyx:302-310), so one tensor sits at two
+# leaves and the count carries its weight sum twice; the oracle reproduces that, and the
+# FP32-scaled executor's leaf factors must follow every leaf occurrence.
+UNUSED_VARIABLE_CNFS = [
+    ("unweighted", "p cnf 5 3\n1 -2 0\n2 3 0\n-1 -3 0\n"),
+    ("cachet", "p cnf 6 3\nw 1 0.3\nw 2 0.7\nw 4 -1\nw 5 0.125\n1 -2 0\n2 3 0\n-1 -3 2 0\n"),
+    ("mcc", "p cnf 5 2\nw 1 0.25\nw -1 0.5\nw 4 3.5\nw -4 0.75\nw 5 2\nw -5 2\n1 -2 0\n2 3 0\n"),
+]
+
+
+def unused_variable_plans():
+    from tensororder_amd import network, planners
+    from tensororder_amd.formula import parse_dimacs_text
+
+    out = []
+    for wf, text in UNUSED_VARIABLE_CNFS:
+        f = parse_dimacs_text(text, wf)
+        for planner in ("line", "factor"):
+            out.append(("%s-%s" % (wf, planner), planners.best_plan(network.cnf_count(f), planner, solver=None)))
+    return out
+
+
+@pytest.mark.parametrize("et,tol", [(_lib.TO_F64, 1e-12), (_lib.TO_F32S, 1e-6)])
+def test_unused_variables_emulated(et, tol):
+    plans = unused_variable_plans()
+    repeated = 0
+    instances, wants = [], []
+    for name, plan in plans:
+        want = numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan))
+        tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
+        leaves = [t.tensor_index for t in tree.iterate_postorder() if t.is_leaf]
+        repeated += len(leaves) - len(set(leaves))
+        got, _ = emulator.emulate_plan(tree, tables, G, total, et, 1)
+        assert got == pytest.approx(want, rel=tol), name
+        instances.append((tree, tables))
+        wants.append(want)
+    # the line plans hold repeated leaves (otherwise this test would not test them)
+    assert repeated > 0
+    got, _ = emulator.emulate_batch(instances, et)
+    for (name, _), g, w in zip(plans, got, wants):
+        assert g == pytest.approx(w, rel=tol), name
+
+
+class _NumpyOps:
+    def create_tensor(self, shape, default_value=None):
+        return np.full(shape, 0.0 if default_value is None else default_value, dtype=np.float64)
+
+    def tensordot(self, a, b, axes):
+        return np.tensordot(a, b, axes)
+
+
+def early_contracted_plan(early, nvars=26, nclauses=60, seed=5, slices=3):
+    """A sliced factor plan after the reference's early contraction
+    (sliced_execution_plan.py:29-56): rerooted, regrouped, joins below ``early`` already
+    contracted into dense leaves."""
+    from tensororder_amd import network, planners
+    from tensororder_amd.formula import parse_dimacs_text
+    from tensororder_amd.instances import random_3cnf
+    from tensororder_amd.plan import ALL_SLICERS
+
+    f = parse_dimacs_text(random_3cnf(nvars, nclauses, seed), "cachet")
+    plan = planners.best_plan(network.cnf_count(f), "factor", solver=None, seed=0)
+    ALL_SLICERS["greedy_mem"].slice_until(plan, slices=slices)
+    want = numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan))
+    plan.contract_small(early, _NumpyOps())
+    return plan, want
+
+
+@pytest.mark.parametrize("early", [4, 6, 9])
+@pytest.mark.parametrize("et,tol", [(_lib.TO_F64, 1e-12), (_lib.TO_F32S, 1e-6)])
+def test_early_contracted_plan_emulated(early, et, tol):
+    plan, want = early_contracted_plan(early)
+    doc = plan_io.plan_to_dict(plan)
+    assert any(t["kind"] == "dense" for t in doc["tensors"])
+    assert numpy_oracle.contract_sliced(doc) == pytest.approx(want, rel=1e-12)
+    tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
+    for W in sorted({1, total}):
+        got, _ = emulator.emulate_plan(tree, tables, G, total, et, W)
+        assert got == pytest.approx(want, rel=tol)

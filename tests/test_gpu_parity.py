@@ -22,7 +22,9 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURES = plan_fixture_paths()
-SMALL = [f for f in FIXTURES if json.load(open(f))["meta"]["maxrank"] <= 20]
+# every fixture with a stored full count (the vertex covers up to n = 200 at rank 28, the
+# sliced 3-CNFs); the 4096-slice 3-CNF stores a slice prefix and has its own test
+FULL = [f for f in FIXTURES if "float64" in json.load(open(f))["meta"]["expected"]]
 F64_TOL = 1e-12
 F32_TOL = 1e-6
 
@@ -57,7 +59,7 @@ def _expected(doc):
     return float(numpy_oracle.contract_sliced(doc))
 
 
-@pytest.mark.parametrize("path", SMALL, ids=[_name(p) for p in SMALL])
+@pytest.mark.parametrize("path", FULL, ids=[_name(p) for p in FULL])
 def test_fixture_fp64(hip, path):
     doc = json.load(open(path))
     _say("case", _name(path), "fp64")
@@ -68,7 +70,7 @@ def test_fixture_fp64(hip, path):
         assert int(got) == int(doc["meta"]["expected"]["bigint"])
 
 
-@pytest.mark.parametrize("path", SMALL, ids=[_name(p) for p in SMALL])
+@pytest.mark.parametrize("path", FULL, ids=[_name(p) for p in FULL])
 def test_fixture_fp32_scaled(hip32, path):
     doc = json.load(open(path))
     _say("case", _name(path), "fp32s")
@@ -344,3 +346,58 @@ def test_tensordot_row_form(hip, hip32, ra, rb, s):
     got32 = np.asarray(hip32.tensordot(a.astype(np.float32), b.astype(np.float32), (la, lb)).numpy(), np.float64)
     np.testing.assert_allclose(got32, np.tensordot(a.astype(np.float32).astype(np.float64),
                                                    b.astype(np.float32).astype(np.float64), (la, lb)), rtol=1e-6)
+
+
+def test_cnf70_4096_slices(hip, hip32):
+    # the 4096-slice random 3-CNF (70 variables, factor-Flow, max rank 25): the FP64
+    # executor on the first slices against the oracle's prefix sum, then all 4096 slices
+    # in both entry types
+    doc = json.load(open(os.path.join(HERE, "golden", "cnf70_factor_s12.json")))
+    pre = doc["meta"]["expected"]["prefix"]
+    plan = plan_io.plan_from_dict(doc)
+    assert len(plan.groups_to_slice) == 12
+    _say("case cnf70 prefix", pre["slices"])
+    got = hip.contract_sliced(plan, num_slice_limit=pre["slices"])
+    assert hip.last_result.slices == pre["slices"]
+    assert got == pytest.approx(pre["float64"], rel=F64_TOL)
+    _say("case cnf70 full fp64")
+    full64 = hip.contract_sliced(plan)
+    assert hip.last_result.slices == 4096
+    _say("case cnf70 full fp32s")
+    full32 = hip32.contract_sliced(plan)
+    assert hip32.last_result.slices == 4096
+    _say("cnf70: fp64 %.17g fp32s %.17g rel %.3g" % (full64, full32, abs(full32 - full64) / abs(full64)))
+    assert full32 == pytest.approx(full64, rel=F32_TOL)
+    assert full64 > pre["float64"]
+
+
+def test_unused_variables(hip, hip32):
+    # DIMACS inputs declaring variables that no clause uses, under both planners: line
+    # plans hold a rank-0 variable tensor at two leaves (the reference joins it at its first
+    # bag and again at the top), and the count carries its weights twice, as the oracle's
+    from test_emulator import unused_variable_plans
+
+    plans = unused_variable_plans()
+    wants = [numpy_oracle.contract_sliced(plan_io.plan_to_dict(p)) for _, p in plans]
+    for api, tol in ((hip, F64_TOL), (hip32, F32_TOL)):
+        for (name, plan), want in zip(plans, wants):
+            _say("case unused variables", name, api.get_entry_size())
+            assert api.contract_sliced(plan) == pytest.approx(want, rel=tol), name
+        exe = api.prepare_batch([p for _, p in plans])
+        try:
+            res, _ = exe.run()
+        finally:
+            exe.close()
+        assert [r.value for r in res] == pytest.approx(wants, rel=tol)
+
+
+@pytest.mark.parametrize("early", [4, 6, 9])
+def test_early_contracted_plans(hip, hip32, early):
+    # plans after the reference's early contraction (dense leaves, rerooted and regrouped
+    # tree, renumbered slice groups) through both executors
+    from test_emulator import early_contracted_plan
+
+    _say("case early contraction", early)
+    plan, want = early_contracted_plan(early, nvars=34, nclauses=76, seed=7, slices=4)
+    assert hip.contract_sliced(plan) == pytest.approx(want, rel=F64_TOL)
+    assert hip32.contract_sliced(plan) == pytest.approx(want, rel=F32_TOL)

@@ -50,3 +50,17 @@ def test_invalid_records_are_rejected_without_launching():
     perm = (ctypes.c_int32 * 2)(0, 0)
     assert lib.to_permute(ctypes.c_void_p(8), ctypes.c_void_p(8), 2, ctypes.addressof(perm), 8, 1,
                           None) == _lib.TO_ERR_INVALID
+
+
+def test_stream_any_descriptors_name_known_forms():
+    # a TO_STREAM_ANY launch runs each descriptor's own form; the host rejects a
+    # descriptor naming a form the kernel does not have instead of launching it
+    import pytest
+
+    from tensororder_amd import schedule
+
+    for kvar in schedule.STREAM_FORMS:
+        schedule.check_stream_form(kvar)
+    for bad in (0, 3, 32, _lib.TO_STREAM_ANY, _lib.TO_STREAM_VEC, _lib.TO_STREAM_ROWS | 4):
+        with pytest.raises(ValueError):
+            schedule.check_stream_form(bad)
