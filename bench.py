@@ -2,16 +2,22 @@
 
 A step is one full contraction of the workload instance (all of its slices), with the
 plan compiled and its inputs resident in HBM before the timed region; the step ends with
-the count read back to the host.  ``value`` is instances/s of the FP32-scaled executor;
-the plain FP64 executor is timed beside it, and both counts are checked against the
-expected float64 count stored with the plan fixture.
+the count read back to the host.  ``value`` is instances/s of the FP32-scaled executor
+over exactly K timed steps after W warm-up steps -- on the plan the learned cost model
+picks among the workload's candidate plans where it has them (the default workload, the
+4096-slice random 3-CNF), else on the workload's own plan.  Beside it, on the reference
+rule's plan: the plain FP64 executor (``speedup_vs_fp64`` = value / its instances/s) and
+the FP32-scaled executor, each timed for up to K steps within ``--side-seconds``; the FP64
+count is checked against the oracle's stored count or slice prefix, the FP32 counts
+against the FP64 count.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-Prints one JSON line.  ``roofline`` covers the kernel class with the most device time in
-one step, timed live with HIP events on the executor's stream; ``cpu_baseline`` is the
-oracle's numpy restatement of the reference executor (same tensordot calls, same plan)
-timed on this host.
+Prints one JSON line.  ``roofline`` covers the kernel symbol with the most device time in
+one headline step, timed live with HIP events on the executor's stream;
+``cpu_baseline`` is the oracle's numpy restatement of the reference executor (same
+tensordot calls, same plan) timed on this host at the default BLAS thread count and at
+one thread.
 """
 
 import argparse
@@ -148,24 +154,36 @@ def per_kernel_times(api, exe):
     return classes, symbols
 
 
-def cpu_baseline(doc, budget_s, full=False):
-    """The oracle executor (numpy float64 tensordot walk of the same plan) on this host.
+def host_description():
+    """nproc, the cores this process may run on, and the CPU model of this host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cores": usable, "cpu_model": model}
 
-    When one full contraction would exceed the budget, the plan is sliced with the
-    reference's greedy_mem slicer until one slice holds <= 2^23 entries, the first slices
-    are timed in product order (the reference's --slice_cutoff) and the time is scaled to
-    all slices of the sliced plan, i.e. to one instance.  The sample is stated."""
-    import threadpoolctl
+
+def _cpu_leg(doc, budget_s, full):
+    """One timing of the oracle executor under the current BLAS thread setting: (instances
+    per second, sample description)."""
+    import itertools
 
     from oracle import numpy_oracle
     from tensororder_amd import plan_io
     from tensororder_amd.plan import ALL_SLICERS
 
-    info = threadpoolctl.threadpool_info()
-    threads = max([i.get("num_threads", 1) for i in info] or [1])
     plan = plan_io.plan_from_dict(doc)
-    small = plan.memory <= (1 << 23) or full
-    if small:
+    groups0 = len(doc["groups_to_slice"])
+    if groups0 == 0 and (plan.memory <= (1 << 23) or full):
         runs = []
         while not runs or (sum(runs) < budget_s and len(runs) < 5):
             t0 = time.perf_counter()
@@ -173,16 +191,17 @@ def cpu_baseline(doc, budget_s, full=False):
             runs.append(time.perf_counter() - t0)
         runs.sort()
         med = runs[len(runs) // 2]
-        return {"value": 1.0 / med, "unit": "instances/s", "cores": threads, "kind": "port",
-                "sample": "%d full contraction(s) of the plan with numpy float64 tensordot "
-                          "(oracle/numpy_oracle.py), median %.3f s, BLAS threads %d"
-                          % (len(runs), med, threads)}
-    while plan.memory > (1 << 23):
-        ALL_SLICERS["greedy_mem"].slice_once(plan)
-    sliced = plan_io.plan_to_dict(plan)
+        return 1.0 / med, "%d full contraction(s) of the plan, median %.3f s" % (len(runs), med)
+    sliced, how = doc, "the plan's own %d slices" % (1 << groups0)
+    if groups0 == 0:
+        # an unsliced plan too large for one host pass: slice it like the reference's
+        # execution.run would on OOM (greedy_mem) until one slice holds <= 2^23 entries
+        while plan.memory > (1 << 23):
+            ALL_SLICERS["greedy_mem"].slice_once(plan)
+        sliced = plan_io.plan_to_dict(plan)
+        how = "the plan sliced by greedy_mem into %d slices (one slice <= 2^23 entries)" % (
+            1 << len(sliced["groups_to_slice"]))
     total = 2 ** len(sliced["groups_to_slice"])
-    import itertools
-
     t0 = time.perf_counter()
     done = 0
     leaves = [numpy_oracle.build_leaf(d) for d in sliced["tensors"]]
@@ -194,10 +213,33 @@ def cpu_baseline(doc, budget_s, full=False):
             break
     dt = time.perf_counter() - t0
     per_instance = dt / done * total
-    return {"value": 1.0 / per_instance, "unit": "instances/s", "cores": threads, "kind": "port",
-            "sample": "plan sliced by greedy_mem into %d slices (one slice <= 2^23 entries); first %d "
-                      "slices timed with the numpy float64 oracle (%.1f s), scaled to all %d slices = "
-                      "%.1f s per instance; BLAS threads %d" % (total, done, dt, total, per_instance, threads)}
+    return 1.0 / per_instance, ("%s; first %d slices in product order timed (%.1f s, leaf build included), "
+                                "scaled to all %d slices = %.1f s per instance"
+                                % (how, done, dt, total, per_instance))
+
+
+def cpu_baseline(doc, budget_s, full=False):
+    """The oracle executor (numpy float64 tensordot walk of the same plan, product-order
+    FP64 slice sum) on this host, twice: BLAS at its default thread count and at one
+    thread (the reference's --thread_limit 1 through threadpoolctl, numpy_apis.py:47-51).
+
+    A bounded sample: one full contraction when it fits the budget, else the first slices
+    in product order (the reference's --slice_cutoff), scaled to one instance."""
+    import numpy  # noqa: F401  (loads the BLAS library threadpoolctl reports)
+    import threadpoolctl
+
+    info = threadpoolctl.threadpool_info()
+    threads = max([i.get("num_threads", 1) for i in info] or [1])
+    host = host_description()
+    value, sample = _cpu_leg(doc, budget_s, full)
+    with threadpoolctl.threadpool_limits(limits=1):
+        value1, sample1 = _cpu_leg(doc, budget_s, full)
+    return {"value": value, "unit": "instances/s", "cores": threads, "kind": "port",
+            "sample": "numpy float64 oracle (oracle/numpy_oracle.py), BLAS threads %d: %s" % (threads, sample),
+            "single_thread": {"value": value1, "unit": "instances/s", "cores": 1,
+                              "sample": "BLAS threads 1: %s" % sample1},
+            "blas": [{k: i.get(k) for k in ("internal_api", "num_threads", "version")} for i in info],
+            **host}
 
 
 def copy_calibration():
@@ -267,36 +309,42 @@ def permute_sweep(rank=26, reps=10):
     print(json.dumps({"permute_sweep": rows, "rank": rank}), flush=True)
 
 
-def learned_choice_run(api, cand_path, args, fp64_value):
-    """The FP32-scaled executor on the plan the trained cost model picks among the
-    workload's candidate plans (decompositions x slice specs); the reference rule's pick is
-    the workload's own plan, timed above."""
+def timed_steps(exe, steps, warmup, max_s=None):
+    """Run ``warmup`` untimed contractions, then time up to ``steps`` contractions (all of
+    them unless ``max_s`` is given: then at least one, stopping once ``max_s`` seconds have
+    passed), bracketed by device synchronisation.  Returns (steps done, seconds, Result)."""
     import torch
 
+    for _ in range(warmup):
+        exe.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        res = exe.run()
+        done += 1
+        if max_s is not None and time.perf_counter() - t0 >= max_s:
+            break
+    torch.cuda.synchronize()
+    return done, time.perf_counter() - t0, res
+
+
+def learned_choice(api, cand_path):
+    """The plan the trained cost model picks among the workload's candidate plans
+    (decompositions x slice specs), and the reference rule's pick (smallest max rank,
+    greedy_mem slices: planning.py:159-162)."""
     from tensororder_amd import costmodel, train_costmodel
 
     model = costmodel.load_default(api)
     t0 = time.perf_counter()
-    plans, labels, ref, doc = train_costmodel.load_candidates(cand_path)
+    plans, labels, ref, _doc = train_costmodel.load_candidates(cand_path)
     choice, preds = costmodel.learned_choice(model, api, plans)
     choose_s = time.perf_counter() - t0
     log("learned choice #%d %s predicted %.3f s (reference #%d predicted %.3f s), choice took %.1f s"
         % (choice, labels[choice], preds[choice], ref, preds[ref], choose_s))
-    exe = api.prepare(plans[choice])
-    for _ in range(args.warmup):
-        exe.run()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        res = exe.run()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    exe.close()
-    log("learned steps", args.steps, "s", round(dt, 4), "count", res.value)
-    return {"value": args.steps / dt, "ms_per_step": dt / args.steps * 1e3, "candidate": choice,
-            "label": labels[choice], "candidates": len(plans), "reference_candidate": ref,
-            "predicted_s": preds[choice], "predicted_reference_s": preds[ref], "choice_host_s": choose_s,
-            "rel_err_vs_fp64_reference_plan": abs(res.value - fp64_value) / abs(fp64_value)}
+    info = {"candidate": choice, "label": labels[choice], "candidates": len(plans), "reference_candidate": ref,
+            "predicted_s": preds[choice], "predicted_reference_s": preds[ref], "choice_host_s": choose_s}
+    return plans[choice], plans[ref], info
 
 
 def batch_bench(args):
@@ -384,8 +432,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="vc200_weighted_line")
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--workload", default="cnf70_factor_s12")
+    ap.add_argument("--side-seconds", type=float, default=20.0,
+                    help="time budget of each comparison leg (FP64, FP32 on the reference-rule plan)")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-full", action="store_true", help="time the full unsliced oracle contraction")
     ap.add_argument("--fp32-only", action="store_true",
@@ -420,50 +470,64 @@ def main():
 
     path = os.path.join(ROOT, "tests", "golden", args.workload + ".json")
     doc = json.load(open(path))
-    expected = doc["meta"]["expected"].get("float64")
+    exp = doc["meta"]["expected"]
+    expected = exp.get("float64")
     log("workload", args.workload, "maxrank", doc["meta"]["maxrank"], "groups", len(doc["groups_to_slice"]))
+    cand_path = os.path.join(ROOT, "tests", "golden", args.workload.rsplit("_", 2)[0] + "_candidates.json")
+    use_learned = os.path.exists(cand_path) and not args.no_learned
 
-    results = {}
-    executors = (("fp32s", HipF32API),) if args.fp32_only else (("fp32s", HipF32API), ("fp64", HipAPI))
-    for name, cls in executors:
+    def leg(cls, plan, steps, warmup, max_s=None, kernels=True, limit=None):
         api = cls()
-        plan = plan_io.plan_from_dict(doc)
-        exe = api.prepare(plan)
-        for _ in range(args.warmup):
-            exe.run()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            res = exe.run()
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        log(name, "steps", args.steps, "s", round(dt, 4), "count", res.value, "log10", res.log10)
-        classes, symbols = per_kernel_times(api, exe)
+        exe = api.prepare(plan, num_slice_limit=limit)
+        done, dt, res = timed_steps(exe, steps, warmup, max_s)
+        log(cls.__name__, "steps", done, "s", round(dt, 4), "count", res.value, "log10", res.log10)
+        classes = symbols = None
+        if kernels:
+            classes, symbols = per_kernel_times(api, exe)
+            log(cls.__name__, "kernels", {k: (round(v["ms"], 3), v["launches"]) for k, v in classes.items()})
         # release this executor's device memory before the next one sizes its waves
         exe.close()
         del exe
         torch.cuda.empty_cache()
-        results[name] = dict(api=api, dt=dt, res=res, classes=classes, symbols=symbols)
-        log(name, "kernels", {k: (round(v["ms"], 3), v["launches"]) for k, v in classes.items()})
+        return dict(steps=done, dt=dt, res=res, classes=classes, symbols=symbols, value=done / dt)
 
-    if args.fp32_only:
-        r32 = results["fp32s"]
-        print(json.dumps({"workload": args.workload, "executor": "fp32-scaled", "steps": args.steps,
-                          "ms_per_step": r32["dt"] / args.steps * 1e3, "kernel_symbols": r32["symbols"]}), flush=True)
-        return
-    r32, r64 = results["fp32s"], results["fp64"]
+    # headline: the FP32-scaled executor, on the plan the learned cost model picks where the
+    # workload has candidate plans (else on the workload's own plan): exactly K timed steps
     learned = None
-    cand_path = os.path.join(ROOT, "tests", "golden", args.workload.rsplit("_", 2)[0] + "_candidates.json")
-    if os.path.exists(cand_path) and not args.no_learned:
-        learned = learned_choice_run(r32["api"], cand_path, args, r64["res"].value)
-    rel_32_64 = abs(r32["res"].value - r64["res"].value) / abs(r64["res"].value)
-    rel_64_ref = abs(r64["res"].value - expected) / abs(expected) if expected else None
-    rel_32_ref = abs(r32["res"].value - expected) / abs(expected) if expected else None
+    if use_learned:
+        plan32, ref_plan, learned = learned_choice(HipF32API(), cand_path)
+        torch.cuda.empty_cache()
+    else:
+        plan32 = ref_plan = plan_io.plan_from_dict(doc)
+    head = leg(HipF32API, plan32, args.steps, args.warmup)
+    if args.fp32_only:
+        print(json.dumps({"workload": args.workload, "executor": "fp32-scaled", "steps": args.steps,
+                          "ms_per_step": head["dt"] / args.steps * 1e3, "kernel_symbols": head["symbols"]}), flush=True)
+        return
+    # beside it, on the reference rule's plan (the workload's own plan): the plain FP64
+    # executor, and the FP32-scaled executor when the headline ran another plan; these
+    # legs time up to K steps within a bounded time
+    side_warm = min(args.warmup, 1)
+    r64 = leg(HipAPI, ref_plan, args.steps, side_warm, max_s=args.side_seconds, kernels=False)
+    r32ref = None
+    if use_learned:
+        r32ref = leg(HipF32API, ref_plan, args.steps, side_warm, max_s=args.side_seconds, kernels=False)
+    # FP64 executor on the oracle's slice prefix, where the fixture stores one
+    prefix = None
+    if "prefix" in exp:
+        k = exp["prefix"]["slices"]
+        rp = leg(HipAPI, plan_io.plan_from_dict(doc), 1, 0, kernels=False, limit=k)
+        prefix = {"slices": k, "oracle": exp["prefix"]["float64"], "fp64": rp["res"].value,
+                  "rel_err": abs(rp["res"].value - exp["prefix"]["float64"]) / abs(exp["prefix"]["float64"])}
+        log("fp64 prefix", prefix)
 
-    classes, symbols = r32["classes"], r32["symbols"]
+    c64 = r64["res"].value
+    rel_32_64 = abs(head["res"].value - c64) / abs(c64)
+    rel_64_ref = abs(c64 - expected) / abs(expected) if expected else None
+    rel_32_ref = abs(head["res"].value - expected) / abs(expected) if expected else None
+    classes, symbols = head["classes"], head["symbols"]
     dom_name = max(symbols, key=lambda k: symbols[k]["ms"])
-    dom = symbols[dom_name]
-    roofline = roofline_of(dom_name, dom, args.workload)
+    roofline = roofline_of(dom_name, symbols[dom_name], args.workload)
     # the permute kernel's own HBM roofline (it is HBM-bound whatever dominates)
     perm = {k: v for k, v in symbols.items() if k.startswith("k_permute")}
     perm_roofline = None
@@ -471,7 +535,8 @@ def main():
         pname = max(perm, key=lambda k: perm[k]["ms"])
         perm_roofline = roofline_of(pname, perm[pname], args.workload)
 
-    value = args.steps / r32["dt"] if learned is None else learned["value"]
+    value = head["value"]
+    executor = "fp32-scaled" + (", plan chosen by the learned cost model" if learned else "")
     out = {
         "metric": METRIC,
         "value": value,
@@ -486,14 +551,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "description": WORKLOADS.get(args.workload, ""),
-                   "maxrank": doc["meta"]["maxrank"], "slices": 2 ** len(doc["groups_to_slice"]),
-                   "executor": "fp32-scaled" + ("" if learned is None else ", plan chosen by the learned cost model")},
-        "fp64": {"value": args.steps / r64["dt"], "ms_per_step": r64["dt"] / args.steps * 1e3,
-                 "rel_err_vs_oracle": rel_64_ref},
-        "fp32s_reference_plan": {"value": args.steps / r32["dt"], "ms_per_step": r32["dt"] / args.steps * 1e3},
-        "fp32s_speedup_vs_fp64": r64["dt"] / r32["dt"],
+                   "maxrank": plan32.maxrank, "slices": 2 ** len([g for g in plan32.groups_to_slice if g]),
+                   "executor": executor},
         "learned": learned,
-        "speedup_vs_fp64": value / (args.steps / r64["dt"]),
+        "fp64": {"plan": "reference rule", "value": r64["value"], "ms_per_step": r64["dt"] / r64["steps"] * 1e3,
+                 "steps": r64["steps"], "warmup": side_warm, "rel_err_vs_oracle": rel_64_ref,
+                 "prefix_vs_oracle": prefix},
+        "fp32s_reference_plan": None if r32ref is None else {
+            "value": r32ref["value"], "ms_per_step": r32ref["dt"] / r32ref["steps"] * 1e3, "steps": r32ref["steps"],
+            "speedup_vs_fp64": r32ref["value"] / r64["value"],
+            "rel_err_vs_fp64": abs(r32ref["res"].value - c64) / abs(c64)},
+        "speedup_vs_fp64": value / r64["value"],
         "rel_err_fp32s_vs_fp64": rel_32_64,
         "rel_err_fp32s_vs_oracle": rel_32_ref,
         "log10_count": r64["res"].log10,
