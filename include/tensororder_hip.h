@@ -240,6 +240,22 @@ int to_graph_destroy(void* graph_exec);
 int to_permute(const void* in, void* out, int rank, const int32_t* perm, int elem_bytes,
                int64_t batch, void* stream);
 
+/* numpy.tensordot(a, b, (axes_a, axes_b)) of two all-dimension-2 tensors on the device:
+ * the per-join primitive of the reference's post-order walk
+ * (src/tensor_network/tensor_network.pyx:152-154 -> tensor_apis/numpy_apis.py:42-43).
+ * c receives 2^(rank_a + rank_b - 2 nshared) entries in numpy's axis order (a's unshared
+ * axes in order, then b's); ranks <= 30, output rank <= 30; c must not overlap a or b.
+ * TO_F64: float64 tensors, the records are ignored.  TO_F32S: float32 tensors with one
+ * to_scale_rec each (a_rec, b_rec read; c_rec written).  One streamed join, or, for many
+ * shared axes and few outputs, split-K partial sums in double in `workspace` (8-byte
+ * aligned; to_tensordot_workspace gives the bytes, 0 when none are needed) and a
+ * fixed-order finishing sum.  Asynchronous on `stream`. */
+int to_tensordot_workspace(int rank_a, int rank_b, const int32_t* axes_a, const int32_t* axes_b,
+                           int nshared, int entry_type, int64_t* bytes);
+int to_tensordot(const void* a, const void* b, void* c, int rank_a, int rank_b,
+                 const int32_t* axes_a, const int32_t* axes_b, int nshared, int entry_type,
+                 const void* a_rec, const void* b_rec, void* c_rec, void* workspace,
+                 int64_t workspace_bytes, void* stream);
 /* Frequency of the device clock stamped by TO_OP_ROOT_STORE / TO_OP_BATCH_BEGIN, in kHz. */
 int to_wall_clock_khz(int* khz);
 

@@ -416,13 +416,11 @@ __global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __re
 // split-K join: block (o, sp) sums its share of the 2^s shared indices of output o.
 
 template <typename T, bool SCALED>
-__global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __restrict__ descs, int ndesc) {
+__device__ __forceinline__ void join_splitk_body(const to_join_desc* __restrict__ d, const int64_t local) {
     __shared__ double red[kBlock / 64];
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
     const int tid = threadIdx.x;
     const int fx = d->fx, fy = d->fy, s = d->s;
     const int nsplit = d->nsplit;
-    const int64_t local = blockIdx.x - d->block0;
     const int64_t o = local / nsplit;
     const int sp = (int)(local % nsplit);
     const int64_t batch = o >> (fx + fy);
@@ -471,15 +469,20 @@ __global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __re
 }
 
 template <typename T, bool SCALED>
-__global__ __launch_bounds__(kBlock) void k_join_finish(const to_join_desc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __restrict__ descs, int ndesc) {
     const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    join_splitk_body<T, SCALED>(d, blockIdx.x - d->block0);
+}
+
+template <typename T, bool SCALED>
+__device__ __forceinline__ void join_finish_body(const to_join_desc* __restrict__ d, const int64_t local) {
     const int fx = d->fx, fy = d->fy;
     const int nsplit = d->nsplit;
     const int64_t nout = (int64_t)d->nbatch << (fx + fy);
     const TO_GLOBAL double* partial = gptr<const double>(d->partial);
     TO_GLOBAL T* C = gptr<T>(d->c);
     const int64_t stride = (int64_t)d->nblocks * kBlock;
-    for (int64_t o = (int64_t)(blockIdx.x - d->block0) * kBlock + threadIdx.x; o < nout; o += stride) {
+    for (int64_t o = local * kBlock + threadIdx.x; o < nout; o += stride) {
         double acc = 0.0;
         for (int sp = 0; sp < nsplit; ++sp) acc += partial[o * nsplit + sp];
         const T sum = (T)acc;
@@ -497,18 +500,23 @@ __global__ __launch_bounds__(kBlock) void k_join_finish(const to_join_desc* __re
     }
 }
 
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_join_finish(const to_join_desc* __restrict__ descs, int ndesc) {
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    join_finish_body<T, SCALED>(d, blockIdx.x - d->block0);
+}
+
 // Many partial sums per output (nsplit >= 32, e.g. a dot product of two large tensors):
 // one block per output, its threads sum strided partials and reduce in LDS.
 template <typename T, bool SCALED>
-__global__ __launch_bounds__(kBlock) void k_join_finish_wide(const to_join_desc* __restrict__ descs, int ndesc) {
+__device__ __forceinline__ void join_finish_wide_body(const to_join_desc* __restrict__ d, const int64_t local) {
     __shared__ double red[kBlock / 64];
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
     const int fx = d->fx, fy = d->fy;
     const int nsplit = d->nsplit;
     const int64_t nout = (int64_t)d->nbatch << (fx + fy);
     const TO_GLOBAL double* partial = gptr<const double>(d->partial);
     TO_GLOBAL T* C = gptr<T>(d->c);
-    for (int64_t o = blockIdx.x - d->block0; o < nout; o += d->nblocks) {
+    for (int64_t o = local; o < nout; o += d->nblocks) {
         double acc = 0.0;
         for (int sp = threadIdx.x; sp < nsplit; sp += kBlock) acc += partial[o * nsplit + sp];
         acc = block_sum(acc, red);
@@ -527,6 +535,34 @@ __global__ __launch_bounds__(kBlock) void k_join_finish_wide(const to_join_desc*
             }
         }
     }
+}
+
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_join_finish_wide(const to_join_desc* __restrict__ descs, int ndesc) {
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    join_finish_wide_body<T, SCALED>(d, blockIdx.x - d->block0);
+}
+
+// One join given by value (to_tensordot): the kernels above without a descriptor table.
+template <typename T, bool SCALED, int YC>
+__global__ __launch_bounds__(kBlock) void k_join_stream_one(const to_join_desc desc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    join_stream_body<T, SCALED, YC>(&desc, blockIdx.x, smem);
+}
+
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_join_splitk_one(const to_join_desc desc) {
+    join_splitk_body<T, SCALED>(&desc, blockIdx.x);
+}
+
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_join_finish_one(const to_join_desc desc) {
+    join_finish_body<T, SCALED>(&desc, blockIdx.x);
+}
+
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_join_finish_wide_one(const to_join_desc desc) {
+    join_finish_wide_body<T, SCALED>(&desc, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------------
@@ -2400,6 +2436,216 @@ int to_permute(const void* in, void* out, int rank, const int32_t* perm, int ele
         hipLaunchKernelGGL(k_permute<float>, dim3(grid), dim3(kBlock), lds, st, p);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? TO_OK : hip_fail(e, "to_permute launch");
+}
+
+}  // extern "C"
+
+namespace {
+
+// One tensordot as one join: the bit geometry, the kernel form (streamed join, or split-K
+// with double partial sums and a finishing pass) and the scratch it needs.
+struct DotPlan {
+    to_join_desc d;
+    bool x_is_left;  // a is the streamed operand X
+    bool splitk;
+    int yc, lds, grid, fin_grid;
+    bool fin_wide;
+    int64_t partial_bytes;
+};
+
+constexpr int kDotMaxRank = 30;
+constexpr int kDotLdsLimit = 64 * 1024;
+constexpr int kDotMaxBlocks = 4096;
+
+// numpy.tensordot(a, b, (axes_a, axes_b)) of all-dimension-2 tensors: output axes are a's
+// unshared axes in order, then b's (numpy/_core/numeric.py:1161-1178); axis i of a rank-r
+// tensor is bit r-1-i of its flat index.  The operand with more free bits is streamed (X).
+int plan_tensordot(int rank_a, int rank_b, const int32_t* axes_a, const int32_t* axes_b, int nshared,
+                   int entry_type, DotPlan& P) {
+    if (entry_type != TO_F64 && entry_type != TO_F32S) {
+        set_err("tensordot: unknown entry type");
+        return TO_ERR_INVALID;
+    }
+    if (rank_a < 0 || rank_b < 0 || rank_a > kDotMaxRank || rank_b > kDotMaxRank || nshared < 0 ||
+        nshared > rank_a || nshared > rank_b || (nshared > 0 && (!axes_a || !axes_b))) {
+        set_err("tensordot: invalid ranks or shared-axis count");
+        return TO_ERR_INVALID;
+    }
+    bool in_a[32] = {false}, in_b[32] = {false};
+    for (int t = 0; t < nshared; ++t) {
+        const int i = axes_a[t], j = axes_b[t];
+        if (i < 0 || i >= rank_a || j < 0 || j >= rank_b || in_a[i] || in_b[j]) {
+            set_err("tensordot: axes out of range or repeated");
+            return TO_ERR_INVALID;
+        }
+        in_a[i] = in_b[j] = true;
+    }
+    const int s = nshared, na = rank_a - s, nb = rank_b - s;
+    if (na + nb > kDotMaxRank) {
+        set_err("tensordot: output rank above 30");
+        return TO_ERR_INVALID;
+    }
+    int fa[32], fb[32], k = 0;
+    for (int i = 0; i < rank_a; ++i)
+        if (!in_a[i]) fa[k++] = i;
+    k = 0;
+    for (int i = 0; i < rank_b; ++i)
+        if (!in_b[i]) fb[k++] = i;
+    int8_t a_free[32], b_free[32], a_sh[32], b_sh[32];
+    for (int t = 0; t < na; ++t) a_free[t] = (int8_t)(rank_a - 1 - fa[na - 1 - t]);
+    for (int t = 0; t < nb; ++t) b_free[t] = (int8_t)(rank_b - 1 - fb[nb - 1 - t]);
+    for (int t = 0; t < s; ++t) {
+        a_sh[t] = (int8_t)(rank_a - 1 - axes_a[s - 1 - t]);
+        b_sh[t] = (int8_t)(rank_b - 1 - axes_b[s - 1 - t]);
+    }
+    const bool x_is_left = na > nb || (na == nb && rank_a >= rank_b);
+    memset(&P, 0, sizeof(P));
+    to_join_desc& d = P.d;
+    const int fx = x_is_left ? na : nb, fy = x_is_left ? nb : na;
+    d.fx = fx;
+    d.fy = fy;
+    d.s = s;
+    d.shf = x_is_left ? nb : 0;
+    d.shy = x_is_left ? 0 : nb;
+    d.nbatch = 1;
+    memcpy(d.xf_pos, x_is_left ? a_free : b_free, (size_t)fx);
+    memcpy(d.yf_pos, x_is_left ? b_free : a_free, (size_t)fy);
+    memcpy(d.xs_pos, x_is_left ? a_sh : b_sh, (size_t)s);
+    memcpy(d.ys_pos, x_is_left ? b_sh : a_sh, (size_t)s);
+    P.x_is_left = x_is_left;
+    const int esize = entry_type == TO_F64 ? 8 : 4;
+    const int64_t K = 1ll << s;
+    if (s <= 12) {
+        int yc = fy >= 4 ? 16 : 1 << fy;
+        int64_t lds = 8 * K;
+        if (fx >= 8) {
+            while (yc > 1 && 8 * K + yc * K * esize > kDotLdsLimit) yc >>= 1;
+            lds = 8 * K + yc * K * esize;
+        }
+        const int64_t nych = (1ll << fy) / yc;
+        if (lds <= kDotLdsLimit && !(s >= 8 && (nych << fx) < 32768)) {
+            const int64_t ntiles = fx >= 8 ? (nych << (fx - 8)) : ((1ll << fx) + kBlock - 1) / kBlock * nych;
+            P.splitk = false;
+            P.yc = yc;
+            P.lds = (int)lds;
+            P.grid = (int)(ntiles < kDotMaxBlocks ? ntiles : kDotMaxBlocks);
+            int l = 0;
+            while ((1ll << l) < nych) ++l;
+            d.nych = (int32_t)nych;
+            d.lnych = l;
+            d.nsplit = 1;
+            d.nblocks = P.grid;
+            return TO_OK;
+        }
+    }
+    // split-K: block (o, sp) sums its share of the 2^s shared indices of output o
+    const int64_t nout = 1ll << (fx + fy);
+    int lsplit = 0;
+    if (s >= 8) {
+        const int64_t room = nout < 4096 ? 4096 / nout : 1;
+        while ((2ll << lsplit) <= room && lsplit < s - 8) ++lsplit;
+    }
+    const int64_t nsplit = 1ll << lsplit;
+    if (nout * nsplit > 0x7fffffffll) {
+        set_err("tensordot: too many outputs for the split-K form");
+        return TO_ERR_INVALID;
+    }
+    P.splitk = true;
+    P.grid = (int)(nout * nsplit);
+    P.fin_wide = nsplit >= 32;
+    const int64_t fin = P.fin_wide ? nout : (nout + kBlock - 1) / kBlock;
+    P.fin_grid = (int)(fin < kDotMaxBlocks ? fin : kDotMaxBlocks);
+    P.partial_bytes = nout * nsplit * (int64_t)sizeof(double);
+    d.nsplit = (int32_t)nsplit;
+    d.nblocks = P.grid;
+    return TO_OK;
+}
+
+// Launch a planned tensordot: clear the output's scale record (TO_F32S), then the join.
+template <typename T, bool SCALED>
+int launch_tensordot(DotPlan& P, hipStream_t st) {
+    if (SCALED) hipLaunchKernelGGL(k_scale_clear, dim3(1), dim3(kBlock), 0, st, reinterpret_cast<to_scale_rec*>(P.d.cs), (int64_t)1);
+    const dim3 block(kBlock);
+    if (!P.splitk) {
+        switch (P.yc) {
+#define TO_DOT_CASE(YC)                                                                                   \
+    case YC:                                                                                              \
+        hipLaunchKernelGGL((k_join_stream_one<T, SCALED, YC>), dim3(P.grid), block, P.lds, st, P.d);     \
+        break;
+            TO_DOT_CASE(1)
+            TO_DOT_CASE(2)
+            TO_DOT_CASE(4)
+            TO_DOT_CASE(8)
+            TO_DOT_CASE(16)
+#undef TO_DOT_CASE
+            default:
+                set_err("tensordot: bad y chunk");
+                return TO_ERR_INVALID;
+        }
+    } else {
+        hipLaunchKernelGGL((k_join_splitk_one<T, SCALED>), dim3(P.grid), block, 0, st, P.d);
+        to_join_desc fin = P.d;
+        fin.nblocks = P.fin_grid;  // the finishing kernels stride over outputs by their grid
+        if (P.fin_wide)
+            hipLaunchKernelGGL((k_join_finish_wide_one<T, SCALED>), dim3(P.fin_grid), block, 0, st, fin);
+        else
+            hipLaunchKernelGGL((k_join_finish_one<T, SCALED>), dim3(P.fin_grid), block, 0, st, fin);
+    }
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TO_OK : hip_fail(e, "tensordot launch");
+}
+
+int run_tensordot(DotPlan& P, const void* a, const void* b, void* c, int entry_type, const void* a_rec,
+                  const void* b_rec, void* c_rec, void* partial, hipStream_t st) {
+    if (!a || !b || !c || (entry_type == TO_F32S && (!a_rec || !b_rec || !c_rec))) {
+        set_err("tensordot: null tensor or scale record");
+        return TO_ERR_INVALID;
+    }
+    const bool x_is_left = P.x_is_left;
+    P.d.x = reinterpret_cast<uint64_t>(x_is_left ? a : b);
+    P.d.y = reinterpret_cast<uint64_t>(x_is_left ? b : a);
+    P.d.c = reinterpret_cast<uint64_t>(c);
+    if (entry_type == TO_F32S) {
+        P.d.xs = reinterpret_cast<uint64_t>(x_is_left ? a_rec : b_rec);
+        P.d.ys = reinterpret_cast<uint64_t>(x_is_left ? b_rec : a_rec);
+        P.d.cs = reinterpret_cast<uint64_t>(c_rec);
+    }
+    P.d.partial = reinterpret_cast<uint64_t>(partial);
+    return entry_type == TO_F64 ? launch_tensordot<double, false>(P, st) : launch_tensordot<float, true>(P, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int to_tensordot_workspace(int rank_a, int rank_b, const int32_t* axes_a, const int32_t* axes_b, int nshared,
+                           int entry_type, int64_t* bytes) {
+    g_err[0] = 0;
+    if (!bytes) {
+        set_err("to_tensordot_workspace: null output");
+        return TO_ERR_INVALID;
+    }
+    DotPlan P;
+    const int rc = plan_tensordot(rank_a, rank_b, axes_a, axes_b, nshared, entry_type, P);
+    *bytes = rc == TO_OK ? P.partial_bytes : 0;
+    return rc;
+}
+
+int to_tensordot(const void* a, const void* b, void* c, int rank_a, int rank_b, const int32_t* axes_a,
+                 const int32_t* axes_b, int nshared, int entry_type, const void* a_rec, const void* b_rec,
+                 void* c_rec, void* workspace, int64_t workspace_bytes, void* stream) {
+    g_err[0] = 0;
+    DotPlan P;
+    int rc = plan_tensordot(rank_a, rank_b, axes_a, axes_b, nshared, entry_type, P);
+    if (rc != TO_OK) return rc;
+    if (P.partial_bytes > 0 &&
+        (!workspace || workspace_bytes < P.partial_bytes || (reinterpret_cast<uintptr_t>(workspace) & 7))) {
+        snprintf(g_err, sizeof(g_err), "to_tensordot: needs an 8-byte aligned workspace of %lld bytes",
+                 (long long)P.partial_bytes);
+        return TO_ERR_INVALID;
+    }
+    return run_tensordot(P, a, b, c, entry_type, a_rec, b_rec, c_rec, workspace,
+                         reinterpret_cast<hipStream_t>(stream));
 }
 
 int to_wall_clock_khz(int* khz) {

@@ -401,3 +401,37 @@ def test_early_contracted_plans(hip, hip32, early):
     plan, want = early_contracted_plan(early, nvars=34, nclauses=76, seed=7, slices=4)
     assert hip.contract_sliced(plan) == pytest.approx(want, rel=F64_TOL)
     assert hip32.contract_sliced(plan) == pytest.approx(want, rel=F32_TOL)
+
+
+# (rank a, rank b, shared): streamed joins with fx >= 8 and fx < 8 and every y chunk,
+# outer and scalar products, and the split-K form (s > 12, and s >= 8 with few outputs)
+C_TENSORDOT_SHAPES = [(0, 0, 0), (3, 0, 0), (0, 5, 0), (4, 4, 4), (6, 3, 0), (9, 5, 3), (12, 10, 6), (14, 14, 10),
+                      (13, 9, 1), (11, 12, 7), (20, 20, 20), (16, 16, 14), (18, 10, 9), (15, 14, 13), (10, 16, 3)]
+
+
+@pytest.mark.parametrize("et", ["f64", "f32s"])
+def test_c_tensordot_matches_numpy(et):
+    # the C entry point to_tensordot (numpy.tensordot on the device, workspace from the
+    # caller) against numpy in float64
+    import torch
+
+    from ctensordot import c_tensordot
+
+    from tensororder_amd import _lib
+
+    lib = _lib.load()
+    stream = torch.cuda.Stream()
+    entry_type = _lib.TO_F64 if et == "f64" else _lib.TO_F32S
+    rng = np.random.default_rng(17)
+    for ra, rb, s in C_TENSORDOT_SHAPES:
+        _say("case c tensordot", et, ra, rb, s)
+        a = rng.uniform(0.5, 2.0, (2,) * ra)
+        b = rng.uniform(0.5, 2.0, (2,) * rb)
+        la = [int(x) for x in rng.permutation(ra)[:s]]
+        lb = [int(x) for x in rng.permutation(rb)[:s]]
+        if et == "f32s":  # the float32 inputs the device sees
+            a = a.astype(np.float32).astype(np.float64)
+            b = b.astype(np.float32).astype(np.float64)
+        want = np.tensordot(a, b, (la, lb))
+        got = c_tensordot(lib, stream, a, b, (la, lb), entry_type)
+        np.testing.assert_allclose(got, want, rtol=1e-12 if et == "f64" else 1e-6)

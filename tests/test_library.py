@@ -64,3 +64,42 @@ def test_stream_any_descriptors_name_known_forms():
     for bad in (0, 3, 32, _lib.TO_STREAM_ANY, _lib.TO_STREAM_VEC, _lib.TO_STREAM_ROWS | 4):
         with pytest.raises(ValueError):
             schedule.check_stream_form(bad)
+
+
+class _Axes:
+    """int32 axis arrays kept alive for the duration of a call."""
+
+    def __init__(self, *lists):
+        self.arrays = [(ctypes.c_int32 * max(1, len(v)))(*v) for v in lists]
+
+    def __getitem__(self, i):
+        return ctypes.addressof(self.arrays[i])
+
+
+def test_tensordot_workspace_sizes_and_argument_checks():
+    # the C tensordot's plan is host logic: workspace sizes and argument errors need no GPU
+    lib = _lib.load()
+    nbytes = ctypes.c_int64(-1)
+
+    def ws(ra, rb, la, lb, et):
+        ax = _Axes(la, lb)
+        rc = lib.to_tensordot_workspace(ra, rb, ax[0], ax[1], len(la), et, ctypes.byref(nbytes))
+        return rc, nbytes.value
+
+    # a streamed join needs no workspace
+    assert ws(12, 10, [1, 4], [0, 9], _lib.TO_F64) == (_lib.TO_OK, 0)
+    # a dot product of two rank-20 tensors: split-K, 4096 double partial sums of one output
+    assert ws(20, 20, list(range(20)), list(range(20)), _lib.TO_F32S) == (_lib.TO_OK, 4096 * 8)
+    # 16 outputs, 14 shared axes: 64 splits each
+    assert ws(16, 16, list(range(2, 16)), list(range(14)), _lib.TO_F64) == (_lib.TO_OK, 16 * 64 * 8)
+    for ra, rb, la, lb, et in ((3, 3, [0, 0], [1, 2], _lib.TO_F64),   # repeated axis
+                               (3, 3, [3], [0], _lib.TO_F64),         # axis out of range
+                               (31, 2, [0], [0], _lib.TO_F64),        # rank above 30
+                               (20, 20, [0], [0], _lib.TO_F64),       # output rank 38
+                               (3, 3, [0], [0], 7)):                  # entry type
+        assert ws(ra, rb, la, lb, et)[0] == _lib.TO_ERR_INVALID
+    # a split-K tensordot without its workspace is rejected before any launch
+    ax = _Axes(list(range(20)), list(range(20)))
+    rc = lib.to_tensordot(ctypes.c_void_p(256), ctypes.c_void_p(256), ctypes.c_void_p(256), 20, 20, ax[0], ax[1],
+                          20, _lib.TO_F64, None, None, None, None, 0, None)
+    assert rc == _lib.TO_ERR_INVALID and b"workspace" in lib.to_last_error()
