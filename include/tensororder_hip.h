@@ -256,6 +256,16 @@ int to_tensordot(const void* a, const void* b, void* c, int rank_a, int rank_b,
                  const int32_t* axes_a, const int32_t* axes_b, int nshared, int entry_type,
                  const void* a_rec, const void* b_rec, void* c_rec, void* workspace,
                  int64_t workspace_bytes, void* stream);
+/* to_tensordot without the workspace arguments, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); it has no counterpart in the reference.  It allocates nothing: the split-K
+ * partial sums go to a file-scope device array of 8 MiB (1M doubles) that each GPU holds
+ * until the process ends; a call needing more (to_tensordot_workspace > 8 MiB) returns
+ * TO_ERR_INVALID.  Every call on one GPU shares that GPU's array: calls that overlap on
+ * one GPU (other streams, other threads) race and must be serialised by the caller. */
+int to_tensordot_noalloc(const void* a, const void* b, void* c, int rank_a, int rank_b,
+                         const int32_t* axes_a, const int32_t* axes_b, int nshared, int entry_type,
+                         const void* a_rec, const void* b_rec, void* c_rec, void* stream);
 /* Frequency of the device clock stamped by TO_OP_ROOT_STORE / TO_OP_BATCH_BEGIN, in kHz. */
 int to_wall_clock_khz(int* khz);
 

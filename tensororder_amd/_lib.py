@@ -120,7 +120,7 @@ EXPORTED = [
     "to_version", "to_struct_sizes", "to_last_error", "to_device_count", "to_set_device",
     "to_run", "to_run_timed", "to_graph_create", "to_graph_launch", "to_graph_destroy", "to_permute",
     "to_wall_clock_khz", "to_synchronize", "to_mlp_forward", "to_mlp_backward", "to_mlp_adam",
-    "to_tensordot_workspace", "to_tensordot",
+    "to_tensordot_workspace", "to_tensordot", "to_tensordot_noalloc",
 ]
 MLP_IN, MLP_H1, MLP_H2, MLP_OUT = 16, 64, 64, 2
 MLP_NPARAMS = 64 * 16 + 64 + 64 * 64 + 64 + 2 * 64 + 2
@@ -159,6 +159,7 @@ def load(path=LIB_PATH):
     lib.to_synchronize.argtypes = [vp]
     lib.to_tensordot_workspace.argtypes = [i32, i32, vp, vp, i32, i32, p64]
     lib.to_tensordot.argtypes = [vp, vp, vp, i32, i32, vp, vp, i32, i32, vp, vp, vp, vp, i64, vp]
+    lib.to_tensordot_noalloc.argtypes = [vp, vp, vp, i32, i32, vp, vp, i32, i32, vp, vp, vp, vp]
     lib.to_mlp_forward.argtypes = [vp, vp, i64, vp, vp, vp, vp]
     lib.to_mlp_backward.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp]
     lib.to_mlp_adam.argtypes = [vp, vp, vp, vp, i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,

@@ -16,6 +16,7 @@
 //  * FP32 mode keeps one power-of-two exponent per tensor and batch item; consumers
 //    rescale on load, producers fold max |entry| into an atomicMax on its float bits.
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <type_traits>
 
 #include <stdint.h>
@@ -2614,9 +2615,59 @@ int run_tensordot(DotPlan& P, const void* a, const void* b, void* c, int entry_t
     return entry_type == TO_F64 ? launch_tensordot<double, false>(P, st) : launch_tensordot<float, true>(P, st);
 }
 
+// Scratch of to_tensordot_noalloc: the split-K partial sums (doubles) of one call, in a
+// file-scope device array (every GPU the library runs on holds one copy until the process
+// ends).  Host code never passes the array's name to a kernel: it is a host-side stand-in,
+// not a device address; each device's address comes from hipGetSymbolAddress, once per
+// device, with that device current.
+constexpr int64_t kNoAllocScratchBytes = 8ll << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+int scratch_address(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_err("scratch: device index above 63");
+        return TO_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TO_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int to_tensordot_noalloc(const void* a, const void* b, void* c, int rank_a, int rank_b, const int32_t* axes_a,
+                         const int32_t* axes_b, int nshared, int entry_type, const void* a_rec, const void* b_rec,
+                         void* c_rec, void* stream) {
+    g_err[0] = 0;
+    void* ptr = nullptr;
+    int rc = scratch_address(&ptr);
+    if (rc != TO_OK) return rc;
+    DotPlan P;
+    rc = plan_tensordot(rank_a, rank_b, axes_a, axes_b, nshared, entry_type, P);
+    if (rc != TO_OK) return rc;
+    if (P.partial_bytes > kNoAllocScratchBytes) {
+        snprintf(g_err, sizeof(g_err),
+                 "to_tensordot_noalloc: needs %lld bytes of partial sums; its scratch holds %lld (use to_tensordot)",
+                 (long long)P.partial_bytes, (long long)kNoAllocScratchBytes);
+        return TO_ERR_INVALID;
+    }
+    return run_tensordot(P, a, b, c, entry_type, a_rec, b_rec, c_rec, ptr, reinterpret_cast<hipStream_t>(stream));
+}
 
 int to_tensordot_workspace(int rank_a, int rank_b, const int32_t* axes_a, const int32_t* axes_b, int nshared,
                            int entry_type, int64_t* bytes) {
