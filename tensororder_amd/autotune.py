@@ -65,9 +65,12 @@ def gemm_candidates(esize, mlog, nlog, klog, nb, wide):
             for cap in (256, 512, 1024, 8192):
                 out.append((1 | (2 if wide else 0) | 8, int(min(tiles128, cap))))
             if esize == 4 and schedule.GEMM_X6 and (klog >= 7 or not wide):
-                # float products on the bf16 matrix cores (one block per CU)
+                # float products on the bf16 matrix cores (one block per CU; 256-row tiles
+                # with float accumulators)
+                tall = not wide and mlog >= 8
+                tiles_x6 = tiles128 >> (1 if tall else 0)
                 for cap in (256, 512, 1024, 8192):
-                    out.append((1 | (2 if wide else 0) | 8 | 16, int(min(tiles128, cap))))
+                    out.append((1 | (2 if wide else 0) | 8 | 16 | (32 if tall else 0), int(min(tiles_x6, cap))))
     return sorted(set(out))
 
 
@@ -131,7 +134,7 @@ def tune_gemm(api, mlog, nlog, klog, nb):
         results["%d/%d" % (variant, nblocks)] = _time(api, [L]) / nbt
     best = min(results, key=results.get)
     v, n = (int(x) for x in best.split("/"))
-    v &= 27  # the tile shape; the transposed-output bit is chosen by the output layout
+    v &= 59  # the tile shape; the transposed-output bit is chosen by the output layout
     # the block count scales with the batch: keep it relative to the full tile count
     return {"variant": v, "block_cap": n if nbt == nb else None, "ms_per_item": results[best]}, results
 

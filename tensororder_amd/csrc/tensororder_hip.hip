@@ -1721,49 +1721,77 @@ __device__ __forceinline__ Mfma<float>::V mfma_bf16(U4 a, U4 b, Mfma<float>::V c
                                                   0, 0);
 }
 
-// Waves per block of the split product: 8, a 64x32 quarter-column strip each, two per SIMD
-// (200 VGPRs with the double accumulators).  With 4 waves of 64x64 (one per SIMD) a wave
-// waits on each fragment read with nothing else to issue; 8 waves measured 3-16 % faster
-// (more for the float-accumulator form, which then needed accumulation registers too).
+// Waves per block of the split product: 8, two per SIMD.  With 4 waves (one per SIMD) a
+// wave waits on each fragment read with nothing else to issue; 8 waves measured 3-16 %
+// faster.
 template <bool WIDE>
 constexpr int x6_waves() {
     return 8;
 }
 
-template <bool SCALED, bool WIDE, bool TR>
+// Block tiles of BM x 128 outputs: BM = 128 gives each wave a 64x32 part (float or wide
+// accumulators); BM = 256 a 64x64 part, which reads 30 % fewer LDS bytes and stages 25 %
+// fewer per product and has half the barriers (float accumulators only: the wide form's
+// double accumulators of a 64x64 part do not fit two waves per SIMD).
+// MFMA/VALU interleaving of one product: QN groups of one MFMA and four VALU, then the
+// product's remaining MFMAs
+constexpr int x6_groups(int q, int nt, int tj) {
+    return 6 * ((q + 1) * nt / 4 - q * nt / 4) < 4 * tj ? 6 * ((q + 1) * nt / 4 - q * nt / 4) : 4 * tj;
+}
+
+template <int QN, int TOTAL>
+__device__ __forceinline__ void x6_interleave() {
+#pragma unroll
+    for (int g = 0; g < QN; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+        __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // four VALU
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, TOTAL - QN, 0);
+}
+
+template <bool SCALED, bool WIDE, bool TR, int BM>
 __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ descs, int ndesc) {
-    constexpr int BM = 128, BN = 128, BK = 32;
+    constexpr int BN = 128, BK = 32;
     constexpr int NW = x6_waves<WIDE>();             // waves per block
-    static_assert(NW == 4 || NW == 8, "4 or 8 waves");
-    constexpr int WN = NW / 2;                       // waves along n
-    constexpr int TJ = BN / WN / 16;                 // 16-column MFMA tiles per wave (4 or 2)
-    constexpr int PITCH = BK + 8;                    // bf16 per LDS row
-    constexpr int NV = BM * (BK / 4) / (64 * NW);    // 16-byte vectors per thread and operand (4 or 2)
+    static_assert(NW == 8 && (BM == 128 || (BM == 256 && !WIDE)), "tile shapes");
+    constexpr int WMW = BM / 64;                     // waves along m
+    constexpr int WN = NW / WMW;                     // waves along n
+    constexpr int TJ = BN / WN / 16;                 // 16-column MFMA tiles per wave (2 or 4)
+    constexpr int NVA = BM * (BK / 4) / (64 * NW);   // 16-byte A vectors per thread and slab
+    constexpr int NVB = BN * (BK / 4) / (64 * NW);   // B vectors
+    constexpr int NT = NVA + NVB;
     constexpr int FLUSH = 4;                         // K steps per float partial sum (128 terms)
+    // Chunk-major bf16 planes [k chunk of 8][row][8]: the 16 lanes of a fragment read
+    // (rows r..r+15 of one k chunk) take 256 contiguous bytes, which gfx950 serves without
+    // bank conflicts (row-major rows padded to 80 bytes gave 2-way conflicts).
+    constexpr int PA = (BK / 8) * BM * 16, PB = (BK / 8) * BN * 16;  // bytes per plane
+    constexpr int BUF = 3 * (PA + PB);
     typedef Mfma<float>::V V;
     typedef float L __attribute__((ext_vector_type(4)));
     typedef uint32_t U2 __attribute__((ext_vector_type(2)));
-    __shared__ __align__(16) __bf16 As[2][3][BM][PITCH];
-    __shared__ __align__(16) __bf16 Bs[2][3][BN][PITCH];
+    __shared__ __align__(16) unsigned char smem[2 * BUF];  // [buf][A planes][B planes]
     const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / WN, wn = wave % WN;
     const int64_t K = 1ll << d->klog;
-    const int64_t tiles_m = 1ll << (d->mlog - 7), tiles_n = 1ll << (d->nlog - 7);
+    const int64_t tiles_m = 1ll << (d->mlog - (BM == 256 ? 8 : 7)), tiles_n = 1ll << (d->nlog - 7);
     const int64_t per_batch = tiles_m * tiles_n;
     const int64_t ntiles = (int64_t)d->nbatch * per_batch;
-    // Vector i of a thread: 16 bytes (4 k) of one row of a slab; the 8 lanes of a row take
-    // its 8 vectors (128 contiguous bytes), and lanes 8q..8q+7 of a wave take rows
-    // (q >> 1) + 4 (q & 1) of its 8, so that the 16 lanes of one 8-byte LDS store cycle
-    // write rows r and r + 4, which lie in opposite halves of the banks (pitch 80 bytes).
-    int vrow[NV], vk[NV];
+    // Vector v < NVA is A's vector v, else B's vector v - NVA: 16 bytes (4 k) of row
+    // 16 wave + (lane & 15) + 128 (i >> 1), k = 4 (lane >> 4) + 16 (i & 1), so the 32 lanes
+    // of one 8-byte store pass write 256 contiguous bytes of one k chunk.
+    int vrow[NT], vk[NT], voff[NT];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        vrow[i] = 8 * NW * i + 8 * wave + ((lane >> 4) | (((lane >> 3) & 1) << 2));
-        vk[i] = (lane & 7) * 4;
+    for (int v = 0; v < NT; ++v) {
+        const int i = v < NVA ? v : v - NVA;
+        const int rows = v < NVA ? BM : BN;
+        vrow[v] = 16 * wave + (lane & 15) + 128 * (i >> 1);
+        vk[v] = 4 * (lane >> 4) + 16 * (i & 1);
+        voff[v] = (v < NVA ? 0 : 3 * PA) + ((vk[v] >> 3) * rows + vrow[v]) * 16 + (vk[v] & 7) * 2;
     }
-    // fragment offset of this lane inside a plane: row (lane & 15), k = 8 * (lane >> 4)
-    const int frag = (lane & 15) * PITCH + 8 * (lane >> 4);
+    // this lane's fragment (row lane & 15 of a 16-row group, k chunk lane >> 4) in a plane
+    const int frag_a = ((lane >> 4) * BM + (lane & 15)) * 16 + (wm * 4) * 256;
+    const int frag_b = 3 * PA + ((lane >> 4) * BN + (lane & 15)) * 16 + (wn * TJ) * 256;
     float mx = 0.0f;
     int64_t mx_batch = -1;
     for (int64_t t0 = blockIdx.x - d->block0; t0 < ntiles; t0 += d->nblocks) {
@@ -1782,7 +1810,7 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
             sa = x.f; ea = x.e; sb = y.f; eb = y.e;
         }
         V acc[4][TJ];
-        double wide[4][TJ][4];
+        double wide[WIDE ? 4 : 1][WIDE ? TJ : 1][4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -1790,39 +1818,38 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     acc[i][j][v] = 0.0f;
-                    wide[i][j][v] = 0.0;
+                    if (WIDE) wide[WIDE ? i : 0][WIDE ? j : 0][v] = 0.0;
                 }
         // Software pipeline over K slabs of 32: while K step s multiplies LDS buffer
-        // s & 1, each of the thread's 2 NV fetched vectors of slab s + 1 (NV of A, NV of B)
-        // is split into the other buffer between two products, and its register is refilled
-        // at once from slab s + 2.  A vector is thus split at the same point of every step,
-        // one step after its fetch, and the splitting (VALU) runs beside the MFMAs.
-        // Fetches past the end re-read the last slab (in bounds; what they stage is never
-        // read).  One barrier per step.
-        L rv[2 * NV];
+        // s & 1, each of the thread's NT fetched vectors of slab s + 1 is split into the
+        // other buffer between two products, and its register is refilled at once from
+        // slab s + 2.  A vector is thus split at the same point of every step, one step
+        // after its fetch, and the splitting (VALU) runs beside the MFMAs.  Fetches past
+        // the end re-read the last slab (in bounds; what they stage is never read).  One
+        // barrier per step.
+        L rv[NT];
         auto fetch_vec = [&](int v, int64_t k) {
             k = k < K - BK ? k : K - BK;
-            const TO_GLOBAL float* src = v < NV ? A : B;
-            const int i = v & (NV - 1);
-            rv[v] = *reinterpret_cast<const TO_GLOBAL L*>(src + vrow[i] * K + k + vk[i]);
+            const TO_GLOBAL float* src = v < NVA ? A : B;
+            rv[v] = *reinterpret_cast<const TO_GLOBAL L*>(src + vrow[v] * K + k + vk[v]);
         };
         auto stage_vec = [&](int v, int buf) {
-            const int i = v & (NV - 1);
             uint32_t h0, m0, l0, h1, m1, l1;
             split3(rv[v][0], rv[v][1], h0, m0, l0);
             split3(rv[v][2], rv[v][3], h1, m1, l1);
-            __bf16 (*P)[BM][PITCH] = v < NV ? As[buf] : Bs[buf];
-            *reinterpret_cast<U2*>(&P[0][vrow[i]][vk[i]]) = U2{h0, h1};
-            *reinterpret_cast<U2*>(&P[1][vrow[i]][vk[i]]) = U2{m0, m1};
-            *reinterpret_cast<U2*>(&P[2][vrow[i]][vk[i]]) = U2{l0, l1};
+            unsigned char* P = smem + buf * BUF + voff[v];
+            const int pl = v < NVA ? PA : PB;
+            *reinterpret_cast<U2*>(P) = U2{h0, h1};
+            *reinterpret_cast<U2*>(P + pl) = U2{m0, m1};
+            *reinterpret_cast<U2*>(P + 2 * pl) = U2{l0, l1};
         };
-        // stage the q-th quarter of the vectors (NV / 2) of slab s + 1 into `nbuf` and fetch
-        // them from slab s + 2 (at k2)
+        // stage the q-th share of the vectors of slab s + 1 into `nbuf` and fetch them from
+        // slab s + 2 (at k2)
         auto pipe = [&](int q, int nbuf, int64_t k2) {
 #pragma unroll
-            for (int u = 0; u < NV / 2; ++u) stage_vec(q * (NV / 2) + u, nbuf);
+            for (int v = q * NT / 4; v < (q + 1) * NT / 4; ++v) stage_vec(v, nbuf);
 #pragma unroll
-            for (int u = 0; u < NV / 2; ++u) fetch_vec(q * (NV / 2) + u, k2);
+            for (int v = q * NT / 4; v < (q + 1) * NT / 4; ++v) fetch_vec(v, k2);
         };
         // The six products of LDS buffer `buf`, the large one last (a float accumulator
         // that takes the small terms first rounds less): (l,h) (m,h) (m,m) (h,m) (h,l) (h,h).
@@ -1832,16 +1859,15 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
         // fragments (plus the one being read) are live, not all six.
         auto step = [&](int buf, bool fresh, int64_t k2) {
             const int nbuf = buf ^ 1;
+            const unsigned char* S = smem + buf * BUF;
             U4 fa[4], fb[4], fn[4];
             auto read_a = [&](U4 (&f)[4], int plane) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    f[i] = *reinterpret_cast<const U4*>(&As[buf][plane][wm * 64 + i * 16][0] + frag);
+                for (int i = 0; i < 4; ++i) f[i] = *reinterpret_cast<const U4*>(S + plane * PA + frag_a + i * 256);
             };
             auto read_b = [&](U4 (&f)[4], int plane) {
 #pragma unroll
-                for (int j = 0; j < TJ; ++j)
-                    f[j] = *reinterpret_cast<const U4*>(&Bs[buf][plane][wn * 16 * TJ + j * 16][0] + frag);
+                for (int j = 0; j < TJ; ++j) f[j] = *reinterpret_cast<const U4*>(S + plane * PB + frag_b + j * 256);
             };
             auto product = [&](const U4 (&a)[4], const U4 (&b)[4]) {
 #pragma unroll
@@ -1853,18 +1879,9 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
             read_b(fb, 0);   // B h
             read_a(fa, 2);   // A l
             read_a(fn, 1);   // A m (next)
-            // Interleave the splitting of NV / 2 vectors (about 22 VALU each) with the
-            // 4 TJ MFMAs of one product: while an MFMA runs (16 cycles), the wave issues
-            // the next VALU instructions.
-            auto interleave = [&]() {
-                constexpr int QN = NW == 4 ? 11 : 6;
-#pragma unroll
-                for (int q = 0; q < QN; ++q) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
-                    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // four VALU
-                }
-                __builtin_amdgcn_sched_group_barrier(0x008, 4 * TJ - QN, 0);
-            };
+            // Interleave the splitting of the q-th share of the vectors (about 22 VALU
+            // each) with the 4 TJ MFMAs of one product: while an MFMA runs (16 cycles),
+            // the wave issues the next VALU instructions.
             if (fresh) {     // the first product of a partial sum starts from zero
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
@@ -1876,22 +1893,22 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
                 product(fa, fb);                   // (l,h)
             }
             pipe(0, nbuf, k2);
-            interleave();
+            x6_interleave<x6_groups(0, NT, TJ), 4 * TJ>();
             __builtin_amdgcn_sched_barrier(0);
             read_b(fa, 1);   // B m (next; A l is done)
             product(fn, fb);                       // (m,h)
             pipe(1, nbuf, k2);
-            interleave();
+            x6_interleave<x6_groups(1, NT, TJ), 4 * TJ>();
             __builtin_amdgcn_sched_barrier(0);
             read_a(fb, 0);   // A h (next; B h is done)
             product(fn, fa);                       // (m,m)
             pipe(2, nbuf, k2);
-            interleave();
+            x6_interleave<x6_groups(2, NT, TJ), 4 * TJ>();
             __builtin_amdgcn_sched_barrier(0);
             read_b(fn, 2);   // B l (next; A m is done)
             product(fb, fa);                       // (h,m)
             pipe(3, nbuf, k2);
-            interleave();
+            x6_interleave<x6_groups(3, NT, TJ), 4 * TJ>();
             __builtin_amdgcn_sched_barrier(0);
             read_b(fa, 0);   // B h again (next; B m is done)
             product(fb, fn);                       // (h,l)
@@ -1899,10 +1916,10 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
             product(fb, fa);                       // (h,h)
         };
 #pragma unroll
-        for (int v = 0; v < 2 * NV; ++v) fetch_vec(v, 0);
+        for (int v = 0; v < NT; ++v) fetch_vec(v, 0);
         __syncthreads();  // the previous tile's last slab has been read
 #pragma unroll
-        for (int v = 0; v < 2 * NV; ++v) {
+        for (int v = 0; v < NT; ++v) {
             stage_vec(v, 0);
             fetch_vec(v, BK);
         }
@@ -1923,7 +1940,7 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
 #pragma unroll
                         for (int j = 0; j < TJ; ++j)
 #pragma unroll
-                            for (int v = 0; v < 4; ++v) wide[i][j][v] += (double)acc[i][j][v];
+                            for (int v = 0; v < 4; ++v) wide[WIDE ? i : 0][WIDE ? j : 0][v] += (double)acc[i][j][v];
                 }
                 __syncthreads();
             }
@@ -1939,7 +1956,7 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
                 float out[4];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    out[v] = (WIDE ? (float)wide[i][j][v] : acc[i][j][v]) * sab;
+                    out[v] = (WIDE ? (float)wide[WIDE ? i : 0][WIDE ? j : 0][v] : acc[i][j][v]) * sab;
                     if (SCALED) mx = fmaxf(mx, fabsf(out[v]));
                 }
                 const int64_t mb = tm * BM + wm * 64 + i * 16, nb = tn * BN + wn * 16 * TJ + j * 16;
@@ -1963,9 +1980,9 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
 }
 
 // one block per CU, LDS double-buffered
-template <bool SCALED, bool WIDE, bool TR>
+template <bool SCALED, bool WIDE, bool TR, int BM>
 __global__ __launch_bounds__(64 * x6_waves<WIDE>(), 1) void k_gemm_x6(const to_gemm_desc* __restrict__ descs, int ndesc) {
-    gemm_x6_body<SCALED, WIDE, TR>(descs, ndesc);
+    gemm_x6_body<SCALED, WIDE, TR, BM>(descs, ndesc);
 }
 
 // ---------------------------------------------------------------------------------
@@ -2087,18 +2104,17 @@ int launch_one(const to_launch& L, hipStream_t st) {
                 if constexpr (sizeof(T) == 4) {
                     const to_gemm_desc* gd = reinterpret_cast<const to_gemm_desc*>(L.descs);
                     switch (L.variant) {
-                        case 25:
-                            hipLaunchKernelGGL((k_gemm_x6<SCALED, false, false>), grid, dim3(64 * x6_waves<false>()), 0, st, gd, L.count);
-                            break;
-                        case 27:
-                            hipLaunchKernelGGL((k_gemm_x6<SCALED, true, false>), grid, dim3(64 * x6_waves<true>()), 0, st, gd, L.count);
-                            break;
-                        case 29:
-                            hipLaunchKernelGGL((k_gemm_x6<SCALED, false, true>), grid, dim3(64 * x6_waves<false>()), 0, st, gd, L.count);
-                            break;
-                        case 31:
-                            hipLaunchKernelGGL((k_gemm_x6<SCALED, true, true>), grid, dim3(64 * x6_waves<true>()), 0, st, gd, L.count);
-                            break;
+#define TO_X6_CASE(V, W, TRN, BMT)                                                                              \
+    case V:                                                                                                     \
+        hipLaunchKernelGGL((k_gemm_x6<SCALED, W, TRN, BMT>), grid, dim3(64 * x6_waves<W>()), 0, st, gd, L.count); \
+        break;
+                        TO_X6_CASE(25, false, false, 128)
+                        TO_X6_CASE(27, true, false, 128)
+                        TO_X6_CASE(29, false, true, 128)
+                        TO_X6_CASE(31, true, true, 128)
+                        TO_X6_CASE(57, false, false, 256)
+                        TO_X6_CASE(61, false, true, 256)
+#undef TO_X6_CASE
                         default:  // the split product runs at one block per CU (bit 3)
                             set_err("gemm: unknown bf16 split-product variant");
                             return TO_ERR_INVALID;

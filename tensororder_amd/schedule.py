@@ -310,13 +310,16 @@ def choose_variant(g, nb, esize, launch_bound=False):
             # of a batch step, so that they share a launch
             big = db = x6 = False
             cap = MAX_GEMM_BLOCKS
+        # split products with float accumulators take 256-row tiles (a 64x64 part per wave)
+        # when the rows allow: fewer LDS bytes and barriers per product than 128 rows
+        tall = x6 and not wide and fx >= 8
         tn = 6 if (wide and not db) else 7
-        tiles = nb << ((fx - 7 + fy - tn) if big else (fx - 6 + fy - 6))
+        tiles = nb << ((fx - (8 if tall else 7) + fy - tn) if big else (fx - 6 + fy - 6))
         # float outputs whose columns are contiguous (shy == 0) are written 16 bytes at a
         # time from a transposed product
         tr = big and esize == 4 and g["shy"] == 0
         return dict(kind="gemm", tile=(1 if big else 0) | (2 if wide else 0) | (4 if tr else 0) | (8 if db else 0)
-                    | (16 if x6 else 0),
+                    | (16 if x6 else 0) | (32 if tall else 0),
                     nblocks=int(min(tiles, cap)),
                     perm_x=not _canonical(g["xf"], g["xs"]), perm_y=not _canonical(g["yf"], g["ys"]))
     yc = min(16, 1 << fy)

@@ -204,7 +204,7 @@ def test_gemm_descriptors_fit_their_tiles(et):
     for ra, rb, s in ((13, 13, 5), (14, 13, 6), (16, 14, 4), (18, 17, 11), (12, 12, 6)):
         cases.append(_two_tensor_plan(ra, rb, s, seed=1))
     f32 = et == _lib.TO_F32S
-    seen = 0
+    seen = tall = 0
     for plan in cases:
         tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
         heap = emulator.Heap()
@@ -220,7 +220,12 @@ def test_gemm_descriptors_fit_their_tiles(et):
             if variant & 16:  # bf16 split products: float, 128x128 tiles, one block per CU
                 assert f32 and variant & 1 and variant & 8 and gd.nlog >= 7
                 assert gd.klog >= (7 if variant & 2 else 5)  # wide: whole partial sums of 4 slabs
+            if variant & 32:  # 256-row split products: float accumulators only
+                tall += 1
+                assert variant & 16 and not variant & 2 and gd.mlog >= 8
+            assert variant in (0, 1, 2, 3, 5, 7, 9, 11, 13, 15, 25, 27, 29, 31, 57, 61), variant
     assert seen > 0
+    assert tall > 0 or not f32
 
 
 # DIMACS inputs that declare variables no clause uses.  The line planner matches a rank-0

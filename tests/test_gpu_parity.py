@@ -246,6 +246,8 @@ def _gemm_variants(esize, mlog, nlog, klog):
             out += [11, 15]
     if esize == 4 and mlog >= 7 and nlog >= 7 and klog >= 5:
         out += [25, 29]  # bf16 three-way split products
+        if mlog >= 8:
+            out += [57, 61]  # 256-row tiles (float accumulators)
         if klog >= 7:
             out += [27, 31]  # wide: K in whole partial sums of 4 slabs
     return out
@@ -266,7 +268,8 @@ def test_gemm_every_tile_variant(hip, hip32, et):
     esize = api.get_entry_size()
     dt = np.float64 if esize == 8 else np.float32
     rng = np.random.default_rng(11)
-    shapes = [(7, 7, 4), (8, 7, 6)] if esize == 8 else [(7, 7, 5), (7, 7, 6), (8, 8, 11), (8, 6, 12), (7, 8, 13)]
+    shapes = [(7, 7, 4), (8, 7, 6)] if esize == 8 else [(7, 7, 5), (7, 7, 6), (8, 8, 11), (8, 6, 12), (7, 8, 13),
+                                                        (9, 7, 8)]
     dev = torch.device("cuda", 0)
     for mlog, nlog, klog in shapes:
         nb = 2
@@ -296,7 +299,8 @@ def test_gemm_every_tile_variant(hip, hip32, et):
                     gd.as_, gd.bs, gd.cs = recs.data_ptr(), recs.data_ptr() + nb * rsz, recs.data_ptr() + 2 * nb * rsz
                     gd.asb = gd.bsb = 1
                 gd.mlog, gd.nlog, gd.klog = mlog, nlog, klog
-                tiles = nb * (M // (64 if variant in (0, 2) else 128)) * (N // (64 if variant in (0, 2, 3, 7) else 128))
+                rows = 64 if variant in (0, 2) else (256 if variant & 32 else 128)
+                tiles = nb * (M // rows) * (N // (64 if variant in (0, 2, 3, 7) else 128))
                 gd.nbatch, gd.block0, gd.nblocks = nb, 0, max(1, min(3, tiles))
                 blob = torch.frombuffer(bytearray(bytes(gd)), dtype=torch.uint8).to(dev)
                 torch.cuda.synchronize()
