@@ -76,7 +76,9 @@ def kernel_symbol(L, scaled):
     if op == _lib.TO_OP_JOIN_FINISH:
         return "%s<%s, %s>" % ("k_join_finish_wide" if v == 1 else "k_join_finish", T, S)
     if op == _lib.TO_OP_PERMUTE:
-        return "%s<%s>" % ("k_permute_vec_desc" if v == 1 else "k_permute_desc", T)
+        if v in (1, 3):  # chunk tiles of 13 / 14 bits (float), 12 / 13 (double)
+            return "k_permute_vec_desc<%s, %d>" % (T, (13 if scaled else 12) + (v == 3))
+        return "k_permute_desc<%s>" % T
     names = {_lib.TO_OP_LEAF_GATHER: "k_leaf_gather", _lib.TO_OP_JOIN_SPLITK: "k_join_splitk",
              _lib.TO_OP_JOIN_FINISH: "k_join_finish", _lib.TO_OP_ROOT_REDUCE: "k_root_reduce",
              _lib.TO_OP_ROOT_STORE: "k_root_store"}
@@ -256,7 +258,7 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
 
 
-def permute_sweep(rank=26, reps=10):
+def permute_sweep(rank=26, reps=20):
     """Standalone bit-permutation throughput (to_permute) on rank-``rank`` tensors: for each
     entry size and permutation, algorithmic bytes (read once + write once) per second."""
     import ctypes
@@ -279,6 +281,20 @@ def permute_sweep(rank=26, reps=10):
     for esize, dt in ((4, torch.float32), (8, torch.float64)):
         a = torch.rand(1 << rank, dtype=dt, device="cuda")
         b = torch.empty_like(a)
+        # the achievable ceiling: a device copy of the same bytes (read once, write once)
+        for _ in range(3):
+            b.copy_(a)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            b.copy_(a)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        copy_us = e0.elapsed_time(e1) * 1e3 / reps
+        copy_gbs = 2 * esize * (1 << rank) / (copy_us * 1e-6) / 1e9
+        rows.append({"entry_bytes": esize, "perm": "copy (ceiling)", "us": copy_us, "GB/s": copy_gbs,
+                     "frac": copy_gbs / HBM_PEAK_GBS, "correct": True})
+        log(rows[-1])
         for name, perm in perms.items():
             p = (ctypes.c_int32 * rank)(*perm)
 
@@ -304,7 +320,7 @@ def permute_sweep(rank=26, reps=10):
             ok = bool(torch.equal(b, a[src]))
             del j, src
             rows.append({"entry_bytes": esize, "perm": name, "us": us, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS,
-                         "correct": ok})
+                         "frac_of_copy": gbs / copy_gbs, "correct": ok})
             log(rows[-1])
     print(json.dumps({"permute_sweep": rows, "rank": rank}), flush=True)
 

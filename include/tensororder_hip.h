@@ -67,7 +67,8 @@ extern "C" {
 #define TO_OP_STEP_BEGIN 8   /* aux2: to_accum* (zeroed), aux3: wave params (params[0] = 0) */
 #define TO_OP_PERMUTE 9      /* descs: to_perm_desc[count]; variant 1: 16-byte chunk kernel,
                                 which needs nu == 13 (float) / 12 (double) and 16-byte
-                                aligned buffers and strides; variant 0: nu <= 12 */
+                                aligned buffers and strides; variant 3: the same with
+                                nu == 14 / 13 (64 KB tiles); variant 0: nu <= 12 */
 #define TO_OP_GEMM 10        /* descs: to_gemm_desc[count]; variant bit 0: 128-row output
                                 tiles (mlog >= 7), 128 columns (nlog >= 7) or, with bit 1,
                                 64 columns (nlog >= 6), instead of 64x64 (>= 6); bit 1: float

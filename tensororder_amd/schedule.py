@@ -450,16 +450,19 @@ def perm_plan(rank, out_pos, max_tile_bits=12):
     return u_in, v_out, v2u, rest, [out_pos[p] for p in rest]
 
 
-# tile bits of the vectorised permute kernel (32 KB tiles of 16-byte chunks)
+# tile bits of the vectorised permute kernel: 32 KB tiles of 16-byte chunks (variant 1), or
+# 64 KB (variant 3) for tensors with more bits than that
 PERM_VEC_BITS = {4: 13, 8: 12}
 
 
 def perm_desc(src, dst, rank, out_pos, nbatch, stride, esize):
-    """(to_perm_desc, grid, LDS bytes, variant) of a batched bit permutation; variant 1 is
-    the vectorised kernel (tensors of at least its tile bits, 16-byte aligned buffers)."""
+    """(to_perm_desc, grid, LDS bytes, variant) of a batched bit permutation; variants 1
+    and 3 are the vectorised kernel (tensors of at least its tile bits, 16-byte aligned
+    buffers) with 32 KB and 64 KB tiles."""
     vec_bits = PERM_VEC_BITS[esize]
     vec = rank >= vec_bits and src % 16 == 0 and dst % 16 == 0 and (stride * esize) % 16 == 0
-    u_in, v_out, v2u, rest_in, rest_out = perm_plan(rank, out_pos, vec_bits if vec else 12)
+    big = vec and rank > vec_bits
+    u_in, v_out, v2u, rest_in, rest_out = perm_plan(rank, out_pos, (vec_bits + big) if vec else 12)
     d = _lib.PermDesc()
     d.in_, d.out = src, dst
     d.inb = d.outb = stride
@@ -469,15 +472,15 @@ def perm_desc(src, dst, rank, out_pos, nbatch, stride, esize):
             arr[i] = x
     ntiles = nbatch << len(rest_in)
     d.block0 = 0
-    # the chunk kernel holds 4 blocks per CU (36 KB of LDS each): by rule one resident wave
-    # of blocks, each walking a contiguous range of tiles
+    # the chunk kernel holds 2 or 4 blocks per CU (68 or 36 KB of LDS each): by rule 1024
+    # blocks, each walking a contiguous range of tiles
     cap = 1024 if vec else 4096
     tuned = TUNED.get(perm_key(esize, rank, nbatch)) if vec else None
     if tuned is not None:
         cap = tuned["block_cap"]
     d.nblocks = int(min(ntiles, cap))
     lds = ((esize << len(u_in)) + 15) // 16 * 16 + 2 * 256 * 8
-    return d, d.nblocks, lds, 1 if vec else 0
+    return d, d.nblocks, lds, (3 if big else 1) if vec else 0
 
 
 def leaf_encoding(rows):
