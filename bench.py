@@ -383,6 +383,8 @@ def batch_bench(args):
     out = {}
     for name, cls in (("fp32s", HipF32API), ("fp64", HipAPI)):
         api = cls()
+        if args.mem_limit:
+            api.add_argument("mem_limit", args.mem_limit)
         exe = api.prepare_batch(plans)
         for _ in range(args.warmup):
             exe.run()
@@ -396,14 +398,14 @@ def batch_bench(args):
         rel = max(errs)
         worst = sorted(range(len(errs)), key=lambda i: -errs[i])[:5]
         lat_ms = np.asarray(lat) * 1e3
-        prog = exe.prog
-        arr = _lib.launch_array(prog.launches)
-        ms = (ctypes.c_float * len(prog.launches))()
-        _lib.check(api._lib.to_run_timed(ctypes.addressof(arr), len(prog.launches), api._entry_type,
+        launches = exe.launches
+        arr = _lib.launch_array(launches)
+        ms = (ctypes.c_float * len(launches))()
+        _lib.check(api._lib.to_run_timed(ctypes.addressof(arr), len(launches), api._entry_type,
                                          ctypes.c_void_p(api._stream.cuda_stream), ms), "to_run_timed")
         scaled = api._entry_type == _lib.TO_F32S
         symbols = {}
-        for L, (b, fl), t in zip(prog.launches, prog.work, ms):
+        for L, (b, fl), t in zip(launches, exe.work, ms):
             c = symbols.setdefault(kernel_symbol(L, scaled), {"ms": 0.0, "launches": 0, "bytes": 0, "flops": 0})
             c["ms"] += t
             c["launches"] += 1
@@ -417,8 +419,9 @@ def batch_bench(args):
                      "worst_instances": [{"index": i, "n": doc["instances"][i]["n"],
                                           "maxrank": doc["instances"][i]["maxrank"], "rel_err": errs[i]}
                                          for i in worst],
-                     "launches_per_step": len(prog.launches),
-                     "steps_per_batch": prog.stats["steps"], "device_ms_per_step": float(sum(ms)),
+                     "launches_per_step": len(launches), "admission_waves": len(exe.progs),
+                     "steps_per_batch": sum(p.stats["steps"] for p in exe.progs),
+                     "device_ms_per_step": float(sum(ms)),
                      "kernel_symbols": symbols}
         log(name, "batch step %.3f ms, %.1f instances/s, p50 %.3f ms p99 %.3f ms, max rel err %.2e"
             % (out[name]["ms_per_step"], out[name]["value"], out[name]["latency_ms_p50"],
@@ -452,6 +455,8 @@ def main():
     ap.add_argument("--side-seconds", type=float, default=20.0,
                     help="time budget of each comparison leg (FP64, FP32 on the reference-rule plan)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--mem-limit", type=int, default=None,
+                    help="device byte budget of the executor (a batch over it runs in admission waves)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-full", action="store_true", help="time the full unsliced oracle contraction")
     ap.add_argument("--fp32-only", action="store_true",

@@ -29,13 +29,11 @@ class BatchProgram:
         self.stats = {}
 
 
-def compile_batch(instances, entry_type, device_alloc, regroup=True):
-    """``instances``: list of (tree, tables) of unsliced plans.  Returns a BatchProgram."""
+def analyse_batch(instances, entry_type, regroup=True):
+    """Per instance of a list of (tree, tables) of unsliced plans: its analysed node list
+    (joins carry their kernel forms), memory plan and arena peak in bytes."""
     esize = 8 if entry_type == _lib.TO_F64 else 4
-    scaled = entry_type == _lib.TO_F32S
-    fdtype = np.float64 if esize == 8 else np.float32
     per = []
-    arena_top = 0
     for tree, tables in instances:
         nodes = schedule.analyse(tree, tables, regroup)
         if any(n.batched for n in nodes):
@@ -47,11 +45,53 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
                 g["var"] = schedule.choose_variant(g, 1, esize, launch_bound=True)
                 n.geom = g
         peak, scratch = schedule.plan_memory(nodes, tables, 1, esize, "level")
-        per.append(dict(nodes=nodes, tables=tables, base=arena_top, scratch=scratch, peak=peak))
-        arena_top += (max(peak, ALIGN) + ALIGN - 1) // ALIGN * ALIGN
+        per.append(dict(nodes=nodes, tables=tables, scratch=scratch, peak=peak,
+                        region=(max(peak, ALIGN) + ALIGN - 1) // ALIGN * ALIGN,
+                        fixed=sum(t.rows.size for t in tables.values()) * esize + 1024 * len(nodes)))
+    return per
+
+
+def admission_waves(per, budget):
+    """Instances (in order) grouped into waves whose arena regions plus fixed bytes (leaf
+    tables, descriptors) fit ``budget`` bytes; the waves run one after another in one arena
+    of the largest wave's size.  An instance that does not fit alone yields None (the
+    caller slices it)."""
+    waves, cur, used = [], [], 0
+    for i, inst in enumerate(per):
+        need = inst["region"] + inst["fixed"]
+        if need > budget:
+            return None
+        if cur and used + need > budget:
+            waves.append(cur)
+            cur, used = [], 0
+        cur.append(i)
+        used += need
+    if cur:
+        waves.append(cur)
+    return waves
+
+
+def wave_arena_bytes(per, wave):
+    return sum(per[i]["region"] for i in wave)
+
+
+def compile_batch(instances, entry_type, device_alloc, regroup=True, per=None, arena=None):
+    """``instances``: list of (tree, tables) of unsliced plans.  Returns a BatchProgram.
+    ``per`` (from ``analyse_batch``) may be given instead of analysing ``instances`` again,
+    and ``arena`` (an object with ``.data_ptr()`` of at least ``wave_arena_bytes`` bytes)
+    instead of allocating one."""
+    esize = 8 if entry_type == _lib.TO_F64 else 4
+    scaled = entry_type == _lib.TO_F32S
+    fdtype = np.float64 if esize == 8 else np.float32
+    if per is None:
+        per = analyse_batch(instances, entry_type, regroup)
+    arena_top = 0
+    for inst in per:
+        inst["base"] = arena_top
+        arena_top += inst["region"]
 
     prog = BatchProgram()
-    prog.count = len(instances)
+    prog.count = len(per)
     # leaf tables
     chunks, off = [], 0
     for inst in per:
@@ -90,7 +130,7 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True):
                     n.rec, n.rec_stride = nrec, 0
                     nrec += 1
     prog.table = device_alloc(table_host.nbytes)
-    prog.arena = device_alloc(max(ALIGN, arena_top))
+    prog.arena = arena if arena is not None else device_alloc(max(ALIGN, arena_top))
     prog.recs = device_alloc(max(1, nrec) * ctypes.sizeof(_lib.ScaleRec))
     prog.accum = device_alloc(len(per) * ctypes.sizeof(_lib.Accum))
     prog.start = device_alloc(64)

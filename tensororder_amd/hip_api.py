@@ -148,44 +148,60 @@ class Executable:
 
 
 class BatchExecutable:
-    """A batch of unsliced plans resident on the device; ``run()`` contracts all of them
-    (one HIP graph) and returns their Results and per-instance completion times (s from
-    the batch start, device clock)."""
+    """A batch of unsliced plans resident on the device, admitted in waves sized to the
+    device budget: one grouped schedule (one HIP graph) per wave, the waves run in order
+    through one shared arena.  ``run()`` contracts all of them and returns their Results and
+    per-instance completion times (s from the batch start, device clock)."""
 
-    def __init__(self, api, prog):
+    def __init__(self, api, progs, arena=None):
         self.api = api
-        self.prog = prog
-        self._graph = None
+        self.progs = list(progs)
+        self.prog = self.progs[0]
+        self.arena = arena
+        self.count = sum(p.count for p in self.progs)
+        self._graphs = []
         if api._use_graph:
-            arr = _lib.launch_array(prog.launches)
-            graph = ctypes.c_void_p()
-            _lib.check(api._lib.to_graph_create(ctypes.addressof(arr), len(prog.launches), api._entry_type,
-                                                ctypes.c_void_p(api._stream.cuda_stream), ctypes.byref(graph)),
-                       "to_graph_create")
-            self._graph = graph
+            for prog in self.progs:
+                arr = _lib.launch_array(prog.launches)
+                graph = ctypes.c_void_p()
+                _lib.check(api._lib.to_graph_create(ctypes.addressof(arr), len(prog.launches), api._entry_type,
+                                                    ctypes.c_void_p(api._stream.cuda_stream), ctypes.byref(graph)),
+                           "to_graph_create")
+                self._graphs.append(graph)
         khz = ctypes.c_int(0)
         _lib.check(api._lib.to_wall_clock_khz(ctypes.byref(khz)), "to_wall_clock_khz")
         self.clock_hz = khz.value * 1e3
 
+    @property
+    def launches(self):
+        return [L for prog in self.progs for L in prog.launches]
+
+    @property
+    def work(self):
+        return [w for prog in self.progs for w in prog.work]
+
     def enqueue(self):
-        if self._graph is not None:
-            _lib.check(self.api._lib.to_graph_launch(self._graph, ctypes.c_void_p(self.api._stream.cuda_stream)),
-                       "to_graph_launch")
+        if self._graphs:
+            for graph in self._graphs:
+                _lib.check(self.api._lib.to_graph_launch(graph, ctypes.c_void_p(self.api._stream.cuda_stream)),
+                           "to_graph_launch")
         else:
-            self.api._run(self.prog.launches)
+            for prog in self.progs:
+                self.api._run(prog.launches)
 
     def results(self):
         api = self.api
         api.synchronize()
-        n = self.prog.count
         size = ctypes.sizeof(_lib.Accum)
+        res, ends = [], []
         with api._torch.cuda.stream(api._stream):
-            raw = self.prog.accum[: n * size].cpu().numpy().tobytes()
-            start = int(self.prog.start[:8].cpu().numpy().view(np.int64)[0])
-        accs = [_lib.Accum.from_buffer_copy(raw[i * size:(i + 1) * size]) for i in range(n)]
-        res = [Result(a.mant * fm, a.exp + fe, a.count)
-               for a, (fm, fe) in zip(accs, getattr(self.prog, "leaf_factors", [(1.0, 0)] * n))]
-        lat = [(a.pad - start) / self.clock_hz for a in accs]
+            start = int(self.progs[0].start[:8].cpu().numpy().view(np.int64)[0])
+            for prog in self.progs:
+                raw = prog.accum[: prog.count * size].cpu().numpy().tobytes()
+                accs = [_lib.Accum.from_buffer_copy(raw[i * size:(i + 1) * size]) for i in range(prog.count)]
+                res += [Result(a.mant * fm, a.exp + fe, a.count) for a, (fm, fe) in zip(accs, prog.leaf_factors)]
+                ends += [a.pad for a in accs]
+        lat = [(e - start) / self.clock_hz for e in ends]
         return res, lat
 
     def run(self):
@@ -193,9 +209,9 @@ class BatchExecutable:
         return self.results()
 
     def close(self):
-        if self._graph is not None:
-            self.api._lib.to_graph_destroy(self._graph)
-            self._graph = None
+        for graph in self._graphs:
+            self.api._lib.to_graph_destroy(graph)
+        self._graphs = []
 
 
 class HipAPI(BaseTensorAPI):
@@ -371,7 +387,10 @@ class HipAPI(BaseTensorAPI):
         return total, tree, tables, len(groups)
 
     def prepare_batch(self, plans):
-        """Compile many unsliced plans into one grouped schedule (``batch.compile_batch``)."""
+        """Compile many unsliced plans into grouped schedules (``batch.compile_batch``),
+        admitted in waves whose arenas fit the device budget (``mem_limit`` when set): the
+        waves share one arena and run one after another.  Raises OutOfMemoryError when one
+        instance alone does not fit (slice it first)."""
         from . import batch
 
         instances = []
@@ -380,12 +399,21 @@ class HipAPI(BaseTensorAPI):
             if groups:
                 raise ValueError("prepare_batch takes unsliced plans")
             instances.append((tree, tables))
-        prog = batch.compile_batch(instances, self._entry_type, self._alloc, regroup=self._regroup)
-        self._upload(prog.table, prog.table_host.tobytes())
-        self._upload(prog.blob, prog.blob_host.tobytes())
-        if self._entry_type == _lib.TO_F32S:
-            self._upload(prog.recs, batch.rec_init_bytes(prog))
-        return BatchExecutable(self, prog)
+        per = batch.analyse_batch(instances, self._entry_type, regroup=self._regroup)
+        waves = batch.admission_waves(per, self._budget())
+        if waves is None:
+            raise OutOfMemoryError
+        arena = self._alloc(max(max(batch.wave_arena_bytes(per, w) for w in waves), schedule.ALIGN))
+        progs = []
+        for w in waves:
+            prog = batch.compile_batch(None, self._entry_type, self._alloc, regroup=self._regroup,
+                                       per=[per[i] for i in w], arena=arena)
+            self._upload(prog.table, prog.table_host.tobytes())
+            self._upload(prog.blob, prog.blob_host.tobytes())
+            if self._entry_type == _lib.TO_F32S:
+                self._upload(prog.recs, batch.rec_init_bytes(prog))
+            progs.append(prog)
+        return BatchExecutable(self, progs, arena)
 
     def contract_batch(self, plans):
         """Counts (floats) of many unsliced plans contracted together."""

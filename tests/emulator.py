@@ -310,21 +310,33 @@ class HostOps:
         return np.transpose(t, axes=perm)
 
 
-def emulate_batch(instances, entry_type):
-    """Compile a batch of (tree, tables) and emulate it; returns the per-instance values."""
+def emulate_batch(instances, entry_type, budget=None):
+    """Compile a batch of (tree, tables) and emulate it; returns the per-instance values and
+    the program (with ``budget``: admission waves sharing one arena, run in order, and the
+    list of wave programs)."""
     from tensororder_amd import batch
 
     heap = Heap(1 << 29)
-    prog = batch.compile_batch(instances, entry_type, heap.alloc)
-    heap.write(prog.table, prog.table_host.tobytes())
-    heap.write(prog.blob, prog.blob_host.tobytes())
-    if entry_type == _lib.TO_F32S:
-        heap.write(prog.recs, batch.rec_init_bytes(prog))
-    emu = Emulator(heap, entry_type)
-    emu.run(prog.launches)
+    if budget is None:
+        progs = [batch.compile_batch(instances, entry_type, heap.alloc)]
+    else:
+        per = batch.analyse_batch(instances, entry_type)
+        waves = batch.admission_waves(per, budget)
+        assert waves is not None
+        arena = heap.alloc(max(batch.wave_arena_bytes(per, w) for w in waves))
+        progs = [batch.compile_batch(None, entry_type, heap.alloc, per=[per[i] for i in w], arena=arena)
+                 for w in waves]
     out = []
-    for i in range(prog.count):
-        acc = heap.read_struct(_lib.Accum, prog.accum.data_ptr() + i * ctypes.sizeof(_lib.Accum))
-        fm, fe = prog.leaf_factors[i] if hasattr(prog, "leaf_factors") else (1.0, 0)
-        out.append(math.ldexp(acc.mant * fm, acc.exp + fe))
-    return out, prog
+    for prog in progs:
+        heap.write(prog.table, prog.table_host.tobytes())
+        heap.write(prog.blob, prog.blob_host.tobytes())
+        if entry_type == _lib.TO_F32S:
+            heap.write(prog.recs, batch.rec_init_bytes(prog))
+    emu = Emulator(heap, entry_type)
+    for prog in progs:
+        emu.run(prog.launches)
+        for i in range(prog.count):
+            acc = heap.read_struct(_lib.Accum, prog.accum.data_ptr() + i * ctypes.sizeof(_lib.Accum))
+            fm, fe = prog.leaf_factors[i]
+            out.append(math.ldexp(acc.mant * fm, acc.exp + fe))
+    return out, (progs[0] if budget is None else progs)

@@ -309,3 +309,31 @@ def test_early_contracted_plan_emulated(early, et, tol):
     for W in sorted({1, total}):
         got, _ = emulator.emulate_plan(tree, tables, G, total, et, W)
         assert got == pytest.approx(want, rel=tol)
+
+
+@pytest.mark.parametrize("et,tol", [(_lib.TO_F64, 1e-12), (_lib.TO_F32S, 1e-6)])
+def test_batch_admission_waves_emulated(et, tol):
+    # a budget below the whole batch's arena: the instances run in waves, in order, through
+    # one arena of the largest wave's size, and every count still matches the oracle
+    from tensororder_amd import batch, network, planners
+    from tensororder_amd.formula import parse_dimacs_text
+    from tensororder_amd.instances import vertex_cover_cnf
+
+    instances, wants = [], []
+    for seed, n in enumerate([10, 16, 24, 12, 30, 20, 18, 26]):
+        f = parse_dimacs_text(vertex_cover_cnf(n, seed, True), "cachet")
+        plan = planners.best_plan(network.cnf_count(f), "line", solver=None)
+        wants.append(numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan)))
+        tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
+        instances.append((tree, tables))
+    per = batch.analyse_batch(instances, et)
+    need = [p["region"] + p["fixed"] for p in per]
+    budget = max(need) + sum(need) // 3
+    waves = batch.admission_waves(per, budget)
+    assert len(waves) >= 2 and sorted(i for w in waves for i in w) == list(range(len(per)))
+    assert all(sum(need[i] for i in w) <= budget for w in waves)
+    assert batch.admission_waves(per, max(need) - 1) is None
+    got, progs = emulator.emulate_batch(instances, et, budget=budget)
+    assert len(progs) == len(waves)
+    for g, w in zip(got, wants):
+        assert g == pytest.approx(w, rel=tol)

@@ -439,3 +439,36 @@ def test_c_tensordot_matches_numpy(et):
         want = np.tensordot(a, b, (la, lb))
         got = c_tensordot(lib, stream, a, b, (la, lb), entry_type)
         np.testing.assert_allclose(got, want, rtol=1e-12 if et == "f64" else 1e-6)
+
+
+def test_batch_admission_waves(hip32, hip):
+    # the 512-instance fixture under a device budget of about a third of its arena: the
+    # instances are admitted in waves (one graph each, one shared arena), and every count
+    # matches the oracle's
+    from tensororder_amd import batch
+
+    doc = json.load(open(os.path.join(HERE, "golden", "vc_batch512.json")))
+    plans, wants = plan_io.batch_plans(doc)
+    for api, tol in ((hip32, F32_TOL), (hip, F64_TOL)):
+        instances = []
+        for plan in plans:
+            _total, tree, tables, _groups = api._plan_inputs(plan, None)
+            instances.append((tree, tables))
+        per = batch.analyse_batch(instances, api._entry_type)
+        need = [p["region"] + p["fixed"] for p in per]
+        limit = max(max(need), sum(need) // 3)
+        _say("case batch admission waves", api.get_entry_size(), "limit", limit, "of", sum(need))
+        api.add_argument("mem_limit", limit)
+        try:
+            exe = api.prepare_batch(plans)
+        finally:
+            api.add_argument("mem_limit", None)
+        try:
+            assert len(exe.progs) >= 2
+            assert sum(p.count for p in exe.progs) == len(plans)
+            for rep in range(2):  # graph replays
+                res, lat = exe.run()
+                assert [r.value for r in res] == pytest.approx(wants, rel=tol)
+                assert all(l >= 0 for l in lat)
+        finally:
+            exe.close()
