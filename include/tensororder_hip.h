@@ -83,6 +83,8 @@ extern "C" {
                                 31; with bit 1, klog >= 7) */
 #define TO_OP_ROOT_STORE 11  /* descs: to_root_desc[count]: one instance's root -> its accumulator */
 #define TO_OP_BATCH_BEGIN 12 /* aux2: to_accum[iaux0] zeroed; aux3: int64 start time (device clock) */
+#define TO_OP_LEAF_BUILD 13  /* descs: to_leaf_build_desc[count]: leaf tables written from their
+                                compact descriptions (OR / variable leaves) */
 
 #define TO_MAX_BITS 32
 
@@ -126,6 +128,28 @@ typedef struct to_leaf_desc {
     int32_t pad;
 } to_leaf_desc;
 
+/* One leaf table written on the device from its description
+ * (src/tensor_network/tensor_network_constructions.py:44-159).  The table holds the 2^rank
+ * entries of the leaf with its axes reordered (sliced axes first, as tensor.py:95-119
+ * reorders them): table bit q (from the least significant) is bit pos[q] of the leaf's own
+ * C-order index i.  With the bits of i as the axis values (axis a is bit rank-1-a):
+ *   TO_LEAF_VAR: rank 0 -> val0; i == 0 -> val0; i == 2^rank - 1 -> val1; else 0;
+ *   TO_LEAF_OR:  literal bits l = i ^ negmask (negmask: bits of negative literals);
+ *                out_bit < 0: val1 if l != 0 else val0;
+ *                else val1 if bit out_bit of l == (l without that bit != 0) else val0.
+ * val0 / val1 are the values as stored (the weights in float64; their float32 encoding in
+ * TO_F32S). */
+#define TO_LEAF_VAR 0
+#define TO_LEAF_OR 1
+typedef struct to_leaf_build_desc {
+    uint64_t dst;          /* 2^rank entries */
+    double val0, val1;
+    int64_t negmask;
+    int32_t kind, rank, out_bit;
+    int32_t block0, nblocks;
+    int32_t pad;
+    int8_t pos[TO_MAX_BITS];
+} to_leaf_build_desc;
 /* Per-tensor, per-batch-item scale (TO_F32S). */
 typedef struct to_scale_rec {
     int64_t exp;      /* true value = stored value * 2^exp */
@@ -207,7 +231,7 @@ int to_version(void);
 /* Size in bytes of each record type, for host-side layout checks. */
 int to_struct_sizes(int64_t* join_desc, int64_t* leaf_desc, int64_t* scale_rec,
                      int64_t* accum, int64_t* launch, int64_t* perm_desc, int64_t* gemm_desc,
-                     int64_t* root_desc);
+                     int64_t* root_desc, int64_t* leaf_build_desc);
 
 /* Last error message of the calling thread ("" when none). */
 const char* to_last_error(void);

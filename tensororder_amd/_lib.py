@@ -25,14 +25,16 @@ TO_F64, TO_F32S = 0, 1
     TO_OP_GEMM,
     TO_OP_ROOT_STORE,
     TO_OP_BATCH_BEGIN,
-) = range(1, 13)
+    TO_OP_LEAF_BUILD,
+) = range(1, 14)
+TO_LEAF_VAR, TO_LEAF_OR = 0, 1  # leaf build kinds
 TO_STREAM_VEC = 0x100  # stream variant bit: the 16-byte form
 TO_STREAM_ROWS = 0x200  # stream variant: the row form
 TO_STREAM_ANY = 0x400  # stream launch variant: mixed joins, each by its descriptor's kvar
 TO_LAUNCH_SIDE = 0x1  # Launch.flags: captured on the side stream
 TO_LAUNCH_JOIN = 0x2  # Launch.flags: join the side stream into the main one and fork it again first
 MAX_BITS = 32
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 c_i8x32 = ctypes.c_int8 * MAX_BITS
 
@@ -61,6 +63,16 @@ class LeafDesc(ctypes.Structure):
         ("rank", ctypes.c_int32), ("ngroups", ctypes.c_int32),
         ("nbatch", ctypes.c_int32), ("block0", ctypes.c_int32), ("nblocks", ctypes.c_int32),
         ("pad", ctypes.c_int32),
+    ]
+
+
+class LeafBuildDesc(ctypes.Structure):
+    _fields_ = [
+        ("dst", ctypes.c_uint64), ("val0", ctypes.c_double), ("val1", ctypes.c_double),
+        ("negmask", ctypes.c_int64),
+        ("kind", ctypes.c_int32), ("rank", ctypes.c_int32), ("out_bit", ctypes.c_int32),
+        ("block0", ctypes.c_int32), ("nblocks", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("pos", ctypes.c_int8 * 32),
     ]
 
 
@@ -145,7 +157,7 @@ def load(path=LIB_PATH):
     vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
     p64 = ctypes.POINTER(ctypes.c_int64)
     lib.to_version.restype = i32
-    lib.to_struct_sizes.argtypes = [p64] * 8
+    lib.to_struct_sizes.argtypes = [p64] * 9
     lib.to_wall_clock_khz.argtypes = [ctypes.POINTER(ctypes.c_int)]
     lib.to_last_error.restype = ctypes.c_char_p
     lib.to_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
@@ -166,9 +178,10 @@ def load(path=LIB_PATH):
                                 ctypes.c_float, i64, vp]
     if lib.to_version() != ABI_VERSION:
         raise ImportError("libtensororder_hip.so ABI version mismatch; rebuild it")
-    sizes = [ctypes.c_int64() for _ in range(8)]
+    sizes = [ctypes.c_int64() for _ in range(9)]
     lib.to_struct_sizes(*[ctypes.byref(s) for s in sizes])
-    want = [ctypes.sizeof(t) for t in (JoinDesc, LeafDesc, ScaleRec, Accum, Launch, PermDesc, GemmDesc, RootDesc)]
+    want = [ctypes.sizeof(t) for t in (JoinDesc, LeafDesc, ScaleRec, Accum, Launch, PermDesc, GemmDesc, RootDesc,
+                                       LeafBuildDesc)]
     if [s.value for s in sizes] != want:
         raise ImportError("record layout mismatch between library and binding: %s vs %s"
                           % ([s.value for s in sizes], want))

@@ -47,7 +47,7 @@ def analyse_batch(instances, entry_type, regroup=True):
         peak, scratch = schedule.plan_memory(nodes, tables, 1, esize, "level")
         per.append(dict(nodes=nodes, tables=tables, scratch=scratch, peak=peak,
                         region=(max(peak, ALIGN) + ALIGN - 1) // ALIGN * ALIGN,
-                        fixed=sum(t.rows.size for t in tables.values()) * esize + 1024 * len(nodes)))
+                        fixed=sum(t.size for t in tables.values()) * esize + 1024 * len(nodes)))
     return per
 
 
@@ -93,20 +93,28 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True, per=None, a
     prog = BatchProgram()
     prog.count = len(per)
     # leaf tables
-    chunks, off = [], 0
+    chunks, off, builds = [], 0, []
     for inst in per:
         inst["table_off"], inst["exp0"], factor_of = {}, {}, {}
         for n in inst["nodes"]:
             if n.is_leaf and n.tensor not in inst["table_off"]:
-                rows = inst["tables"][n.tensor].rows
+                tab = inst["tables"][n.tensor]
+                if tab.spec is not None:  # written on the device by the leaf build launch
+                    v0, v1, F, e = tab.spec.stored(scaled)
+                    builds.append((tab.spec, off, v0, v1))
+                    size = tab.size
+                else:
+                    rows = tab.rows
+                    if scaled:
+                        rows, F, e = schedule.leaf_encoding(rows)
+                    arr = np.ascontiguousarray(rows[:1], dtype=fdtype).ravel()
+                    chunks.append((off, arr))
+                    size = arr.size
                 if scaled:
-                    rows, F, e = schedule.leaf_encoding(rows)
                     inst["exp0"][n.tensor] = e
                     factor_of[n.tensor] = F
-                arr = np.ascontiguousarray(rows[:1], dtype=fdtype).ravel()
                 inst["table_off"][n.tensor] = off
-                chunks.append((off, arr))
-                off += (arr.size * esize + ALIGN - 1) // ALIGN * ALIGN // esize
+                off += (size * esize + ALIGN - 1) // ALIGN * ALIGN // esize
         # one factor per leaf node (a tensor at two leaves enters the count twice)
         inst["factors"] = [factor_of[n.tensor] for n in inst["nodes"] if n.is_leaf] if scaled else []
     table_host = np.zeros(max(off, 1), dtype=fdtype)
@@ -196,6 +204,17 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True, per=None, a
             blob += bytes(st)
         return start
 
+    # leaf tables written on the device once, right after upload
+    build_off, build_grid = None, 0
+    if builds:
+        bdescs = []
+        for spec, toff, v0, v1 in builds:
+            bd = spec.desc(tbase + toff * esize, v0, v1)
+            bd.block0 = build_grid
+            build_grid += bd.nblocks
+            bdescs.append(bd)
+        build_off = put(bdescs)
+
     pending = []
     pending.append((Launch(op=_lib.TO_OP_BATCH_BEGIN, grid=max(1, min(64, -(-len(per) // 256))),
                            aux2=prog.accum.data_ptr(), aux3=prog.start.data_ptr(), iaux0=len(per)), None,
@@ -234,6 +253,8 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True, per=None, a
     prog.blob_host = np.frombuffer(bytes(blob) or b"\0", dtype=np.uint8).copy()
     prog.blob = device_alloc(prog.blob_host.nbytes)
     bbase = prog.blob.data_ptr()
+    prog.build = [] if build_off is None else [
+        Launch(op=_lib.TO_OP_LEAF_BUILD, count=len(builds), grid=build_grid, descs=bbase + build_off)]
     # per launch: (kind, step) and the algorithmic (bytes, flops) of its joins
     prog.work = []
     for L, off, info, work in pending:

@@ -25,7 +25,7 @@
 
 #include "tensororder_hip.h"
 
-#define TO_ABI_VERSION 5
+#define TO_ABI_VERSION 6
 
 namespace {
 
@@ -180,6 +180,37 @@ __global__ __launch_bounds__(kBlock) void k_leaf_gather(const to_leaf_desc* __re
             r->exp = d->exp0;
             r->maxbits = __float_as_uint(1.0f);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// leaf build: OR and variable leaf tables written from their descriptions, once per plan
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_leaf_build(const to_leaf_build_desc* __restrict__ descs, int ndesc) {
+    const to_leaf_build_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const int rank = d->rank;
+    const int64_t E = 1ll << rank;
+    const int64_t all = E - 1;
+    TO_GLOBAL T* dst = gptr<T>(d->dst);
+    const int64_t stride = (int64_t)d->nblocks * kBlock;
+    for (int64_t e = (int64_t)(blockIdx.x - d->block0) * kBlock + threadIdx.x; e < E; e += stride) {
+        const int64_t i = pdep(e, d->pos, rank);  // the leaf's own C-order index
+        bool one;
+        if (d->kind == TO_LEAF_VAR) {
+            one = rank > 0 && i == all;
+            const double v = one ? d->val1 : ((rank == 0 || i == 0) ? d->val0 : 0.0);
+            dst[e] = (T)v;
+            continue;
+        }
+        const int64_t l = i ^ d->negmask;
+        if (d->out_bit < 0) {
+            one = l != 0;
+        } else {
+            const int64_t ob = 1ll << d->out_bit;
+            one = ((l & ob) != 0) == ((l & ~ob) != 0);
+        }
+        dst[e] = (T)(one ? d->val1 : d->val0);
     }
 }
 
@@ -2147,6 +2178,11 @@ int launch_one(const to_launch& L, hipStream_t st) {
             hipLaunchKernelGGL((k_root_store<T, SCALED>), grid, block, 0, st,
                                reinterpret_cast<const to_root_desc*>(L.descs), L.count);
             break;
+        case TO_OP_LEAF_BUILD:
+            if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
+            hipLaunchKernelGGL((k_leaf_build<T>), grid, block, 0, st,
+                               reinterpret_cast<const to_leaf_build_desc*>(L.descs), L.count);
+            break;
         case TO_OP_BATCH_BEGIN:
             if (!L.aux2 || L.iaux0 <= 0) return TO_ERR_INVALID;
             hipLaunchKernelGGL(k_batch_begin, grid, block, 0, st, reinterpret_cast<to_accum*>(L.aux2), L.iaux0,
@@ -2200,7 +2236,9 @@ extern "C" {
 int to_version(void) { return TO_ABI_VERSION; }
 
 int to_struct_sizes(int64_t* join_desc, int64_t* leaf_desc, int64_t* scale_rec, int64_t* accum,
-                    int64_t* launch, int64_t* perm_desc, int64_t* gemm_desc, int64_t* root_desc) {
+                    int64_t* launch, int64_t* perm_desc, int64_t* gemm_desc, int64_t* root_desc,
+                    int64_t* leaf_build_desc) {
+    if (leaf_build_desc) *leaf_build_desc = sizeof(to_leaf_build_desc);
     if (root_desc) *root_desc = sizeof(to_root_desc);
     if (perm_desc) *perm_desc = sizeof(to_perm_desc);
     if (gemm_desc) *gemm_desc = sizeof(to_gemm_desc);

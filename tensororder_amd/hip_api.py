@@ -214,6 +214,31 @@ class BatchExecutable:
         self._graphs = []
 
 
+def leaf_spec_table(tensor, lookup, ngroups):
+    """The LeafTable of an OR or variable leaf whose table the device writes, with its axes
+    reordered and its row lookup exactly as ``SliceSequence.reordered_tensor`` gives them
+    (``tensor.py:95-119``); None for other leaves."""
+    from .network import OrTensor, VariableTensor
+
+    if isinstance(tensor, VariableTensor):
+        w_neg, w_pos = tensor.weights
+        kind, kw = _lib.TO_LEAF_VAR, dict(w_neg=w_neg, w_pos=w_pos)
+    elif isinstance(tensor, OrTensor):
+        kind, kw = _lib.TO_LEAF_OR, dict(positive=tensor.literals_positive, output_index=tensor.output_index)
+    else:
+        return None
+    rank = tensor.rank
+    sliced = [i for i in range(rank) if i in lookup]
+    unsliced = [i for i in range(rank) if i not in lookup]
+    coef = [0] * ngroups
+    offset = 1
+    for i in reversed(sliced):
+        coef[lookup[i]] += offset
+        offset *= 2
+    spec = schedule.LeafSpec(kind, rank, sliced + unsliced, **kw)
+    return schedule.LeafTable(None, coef, len(unsliced), spec)
+
+
 class HipAPI(BaseTensorAPI):
     def __init__(self, device=0):
         import torch
@@ -231,6 +256,7 @@ class HipAPI(BaseTensorAPI):
         self._use_graph = True
         self._regroup = True
         self._autotune = False
+        self._device_leaves = True
         # launch shapes tuned on this GPU model (autotune.py), used where a shape is known
         from . import autotune
 
@@ -258,6 +284,8 @@ class HipAPI(BaseTensorAPI):
             self._regroup = bool(value)
         elif key == "autotune":
             self._autotune = bool(value)
+        elif key == "device_leaves":
+            self._device_leaves = bool(value)
         else:
             super().add_argument(key, value)
 
@@ -325,7 +353,7 @@ class HipAPI(BaseTensorAPI):
         """Largest slice batch W (<= max_batch) whose arena fits the device budget; raises
         OutOfMemoryError when even one slice does not fit."""
         budget = self._budget()
-        fixed = sum(t.rows.size for t in tables.values()) * self._esize
+        fixed = sum(t.size for t in tables.values()) * self._esize
         W = max(1, min(self._max_batch, total_slices))
         while True:
             peak = schedule.peak_bytes(nodes, tables, W, self._entry_type)
@@ -350,6 +378,7 @@ class HipAPI(BaseTensorAPI):
         self._upload(prog.table, prog.table_host.tobytes())
         self._upload(prog.coef, prog.coef_host.tobytes())
         self._upload(prog.blob, prog.blob_host.tobytes())
+        self._run(prog.build)  # OR / variable leaf tables written on the device
         if self._entry_type == _lib.TO_F32S:
             self._upload(prog.recs, schedule.rec_init_bytes(prog))
         self._upload(prog.params, np.array([0, total_slices, 0, 1], dtype=np.int64).tobytes())
@@ -370,20 +399,32 @@ class HipAPI(BaseTensorAPI):
         return self._compile(tree, tables, groups, total)
 
     def _plan_inputs(self, execution_plan, num_slice_limit):
+        """(slices, slice-free tree, leaf tables, groups) of a plan, as the batched executors
+        take them (``jax_apis.py:253-313``): every leaf reordered to [slice row, unsliced
+        axes] with its row lookup (``tensor_network.pyx:404-442``, ``tensor.py:95-119``).
+        OR and variable leaves are described for the device to write (``device_leaves``);
+        other leaves are built on the host."""
+        network = execution_plan.network
         groups = [g for g in execution_plan.groups_to_slice if len(g) > 0]
-        total, sequences, _assignments = execution_plan.network.get_tensor_slices(
-            execution_plan.groups_to_slice, self.create_tensor
-        )
+        tree = network.remove_sliced_indices_from(execution_plan.tree, execution_plan.groups_to_slice)
+        tables = {}
+        if self._device_leaves:
+            total, lookups = network.slice_lookups(execution_plan.groups_to_slice)
+            for t, lookup in enumerate(lookups):
+                spec_table = leaf_spec_table(network[t], lookup, len(groups))
+                if spec_table is not None:
+                    tables[t] = spec_table
+        if len(tables) < len(network):
+            total, sequences, _assignments = network.get_tensor_slices(execution_plan.groups_to_slice,
+                                                                       self.create_tensor)
+            for t, seq in enumerate(sequences):
+                if t in tables:
+                    continue
+                reordered, lookup, shape = seq.reordered_tensor(self)
+                reordered = np.asarray(reordered, dtype=np.float64)
+                tables[t] = schedule.LeafTable(reordered.reshape(reordered.shape[0], -1), lookup, len(shape))
         if num_slice_limit is not None:
             total = min(total, int(num_slice_limit))
-        tree = execution_plan.network.remove_sliced_indices_from(
-            execution_plan.tree, execution_plan.groups_to_slice
-        )
-        tables = {}
-        for t, seq in enumerate(sequences):
-            reordered, lookup, shape = seq.reordered_tensor(self)
-            reordered = np.asarray(reordered, dtype=np.float64)
-            tables[t] = schedule.LeafTable(reordered.reshape(reordered.shape[0], -1), lookup, len(shape))
         return total, tree, tables, len(groups)
 
     def prepare_batch(self, plans):
@@ -410,6 +451,7 @@ class HipAPI(BaseTensorAPI):
                                        per=[per[i] for i in w], arena=arena)
             self._upload(prog.table, prog.table_host.tobytes())
             self._upload(prog.blob, prog.blob_host.tobytes())
+            self._run(prog.build)  # OR / variable leaf tables written on the device
             if self._entry_type == _lib.TO_F32S:
                 self._upload(prog.recs, batch.rec_init_bytes(prog))
             progs.append(prog)

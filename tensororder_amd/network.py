@@ -526,6 +526,24 @@ class TensorNetwork:
             for t, lookup in zip(self._nodes, lookups)
         ], product(*index_values)
 
+    def slice_lookups(self, edge_groups):
+        """(number of slices, per tensor {axis: slice group}) as ``get_tensor_slices``
+        finds them, without building any tensor."""
+        lookups = [{} for _ in range(len(self._nodes))]
+        total, which = 1, 0
+        for group in edge_groups:
+            if len(group) == 0:
+                continue
+            size = 0
+            for e in group:
+                for k, (t, ax) in enumerate(self._locate(e)):
+                    lookups[t][ax] = which
+                    if k == 0:
+                        size = self._nodes[t].shape[ax]
+            total *= size
+            which += 1
+        return total, lookups
+
     def remove_sliced_indices_from(self, contraction_tree, edge_groups):
         """A copy of the tree whose leaves omit sliced edges (``tensor_network.pyx:444-468``)."""
         from .tree import TreeContext

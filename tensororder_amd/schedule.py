@@ -391,16 +391,78 @@ def choose_variant(g, nb, esize, launch_bound=False):
                 perm_x=scattered(g["xs"]), perm_y=scattered(g["ys"]))
 
 
+class LeafSpec:
+    """An OR or variable leaf of the CNF network (``tensor_network_constructions.py:44-159``)
+    described compactly, so that its table is written on the device (``TO_OP_LEAF_BUILD``)
+    rather than built on the host and uploaded.  ``perm`` lists the leaf's axes in table
+    order: sliced axes first, then the others (``tensor.py:95-119``)."""
+
+    __slots__ = ("kind", "rank", "perm", "w_neg", "w_pos", "positive", "output_index")
+
+    def __init__(self, kind, rank, perm, w_neg=0.0, w_pos=0.0, positive=(), output_index=None):
+        self.kind, self.rank, self.perm = kind, int(rank), [int(a) for a in perm]
+        self.w_neg, self.w_pos = w_neg, w_pos
+        self.positive = [bool(p) for p in positive]
+        self.output_index = output_index
+
+    def values(self):
+        """(val0, val1) as the table holds them (see include/tensororder_hip.h), and the
+        array of the table's distinct values."""
+        if self.kind == _lib.TO_LEAF_VAR:
+            if self.rank == 0:
+                v = float(self.w_neg + self.w_pos)
+                return v, v, np.array([v])
+            return float(self.w_neg), float(self.w_pos), np.array([0.0, self.w_neg, self.w_pos], dtype=np.float64)
+        return 0.0, 1.0, np.array([0.0, 1.0])
+
+    def stored(self, scaled):
+        """(val0, val1, F, e): the values the device writes and, for the FP32-scaled executor,
+        the table's encoding factor and exponent -- those ``leaf_encoding`` gives the whole
+        table, which depend only on its set of values."""
+        v0, v1, vals = self.values()
+        if not scaled:
+            return v0, v1, 1.0, 0
+        enc, F, e = leaf_encoding(np.concatenate([vals, [v0, v1]]))
+        return float(enc[-2]), float(enc[-1]), F, e
+
+    def desc(self, dst, v0, v1):
+        """The ``to_leaf_build_desc`` of this leaf's table at device address ``dst``."""
+        d = _lib.LeafBuildDesc()
+        d.dst, d.val0, d.val1 = dst, v0, v1
+        d.kind, d.rank = self.kind, self.rank
+        r = self.rank
+        for q in range(r):  # table bit q (from the lowest) is axis perm[r-1-q]
+            d.pos[q] = r - 1 - self.perm[r - 1 - q]
+        d.out_bit = -1
+        if self.kind == _lib.TO_LEAF_OR:
+            d.negmask = sum(1 << (r - 1 - a) for a, p in enumerate(self.positive) if not p)
+            if self.output_index is not None:
+                d.out_bit = r - 1 - self.output_index
+        d.nblocks = max(1, min(1024, -(-(1 << r) // 256)))
+        return d
+
+
 class LeafTable:
-    """Per-tensor slice rows: ``rows`` (float64 [nrows, 2^rank]) and ``coef`` (one
-    integer per slice group: row = sum_g coef[g] * assignment[g])."""
+    """Per-tensor slice rows: ``rows`` (float64 [nrows, 2^rank]) -- or, for a leaf written
+    on the device, its ``spec`` -- and ``coef`` (one integer per slice group:
+    row = sum_g coef[g] * assignment[g])."""
 
-    __slots__ = ("rows", "coef", "rank")
+    __slots__ = ("rows", "coef", "rank", "spec", "nrows")
 
-    def __init__(self, rows, coef, rank):
-        self.rows = np.ascontiguousarray(rows, dtype=np.float64).reshape(-1, 1 << rank)
+    def __init__(self, rows, coef, rank, spec=None):
         self.coef = [int(c) for c in coef]
         self.rank = rank
+        self.spec = spec
+        if spec is None:
+            self.rows = np.ascontiguousarray(rows, dtype=np.float64).reshape(-1, 1 << rank)
+            self.nrows = self.rows.shape[0]
+        else:
+            self.rows = None
+            self.nrows = 1 << (spec.rank - rank)
+
+    @property
+    def size(self):
+        return self.nrows << self.rank
 
     @property
     def batched(self):
@@ -777,20 +839,28 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
     # ---- leaf tables (host) -> one device buffer; the scaled executor's leaf factors
     # multiply the count (prog.leaf_factor, mantissa and exponent)
     leaf_ids = sorted({n.tensor for n in nodes if n.is_leaf})
-    table_off, chunks, exp0, factor_of = {}, [], {}, {}
+    table_off, chunks, exp0, factor_of, builds = {}, [], {}, {}, []
     off = 0
     for t in leaf_ids:
-        rows = tables[t].rows
+        tab = tables[t]
+        if tab.spec is not None:  # written on the device by the leaf build launch
+            v0, v1, F, e = tab.spec.stored(scaled)
+            builds.append((t, v0, v1))
+            size = tab.size
+        else:
+            rows = tab.rows
+            if scaled:
+                rows, F, e = leaf_encoding(rows)
+            arr = np.ascontiguousarray(rows, dtype=fdtype).ravel()
+            chunks.append((t, arr))
+            size = arr.size
         if scaled:
-            rows, F, e = leaf_encoding(rows)
             exp0[t] = e
             factor_of[t] = F
-        arr = np.ascontiguousarray(rows, dtype=fdtype).ravel()
         table_off[t] = off
-        chunks.append(arr)
-        off += (arr.size * esize + ALIGN - 1) // ALIGN * ALIGN // esize
+        off += (size * esize + ALIGN - 1) // ALIGN * ALIGN // esize
     table_host = np.zeros(max(off, 1), dtype=fdtype)
-    for t, arr in zip(leaf_ids, chunks):
+    for t, arr in chunks:
         table_host[table_off[t]:table_off[t] + arr.size] = arr
     # one factor per leaf *occurrence*: a tree may hold one tensor at several leaves (the
     # line planner joins a rank-0 tensor at its first bag and again at the top,
@@ -862,6 +932,18 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
         return start
 
     pending = []  # (list, Launch, blob offset attr)
+
+    # leaf tables written on the device once, right after upload
+    prog.build = []
+    if builds:
+        bdescs, block0 = [], 0
+        for t, v0, v1 in builds:
+            bd = tables[t].spec.desc(tbase + table_off[t] * esize, v0, v1)
+            bd.block0 = block0
+            block0 += bd.nblocks
+            bdescs.append(bd)
+        boff = put(bdescs)
+        prog.build.append((Launch(op=_lib.TO_OP_LEAF_BUILD, count=len(bdescs), grid=block0), boff))
 
     def join_launches(n, phase_list):
         for kind, op, variant, lds, st, nblk in join_records(n, nodes, W, esize, scaled, abase,
@@ -971,6 +1053,9 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
     prog.blob_host = np.frombuffer(bytes(blob) or b"\0", dtype=np.uint8).copy()
     prog.blob = device_alloc(prog.blob_host.nbytes)
     bbase = prog.blob.data_ptr()
+    for L, boff in prog.build:
+        L.descs = bbase + boff
+    prog.build = [L for L, _ in prog.build]
     # per launch: (kind, node index) and its algorithmic (bytes, flops): each distinct
     # operand entry read once and each output entry written once
     prog.prologue_info, prog.wave_info = [], []
@@ -991,6 +1076,7 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
         gemm=sum(1 for n in nodes if not n.is_leaf and n.geom["var"]["kind"] == "gemm"),
         arena_bytes=peak,
         table_bytes=int(table_host.nbytes),
+        device_built_leaves=len(builds),
         launches_prologue=len(prog.prologue),
         launches_wave=len(prog.wave),
     )

@@ -249,6 +249,26 @@ class Emulator:
         raw[:] = 0
         self.clock = 1
 
+    def op_13(self, L):  # leaf build: the documented table of each OR / variable leaf
+        size = ctypes.sizeof(_lib.LeafBuildDesc)
+        for i in range(L.count):
+            d = self.heap.read_struct(_lib.LeafBuildDesc, L.descs + i * size)
+            r = d.rank
+            e = np.arange(1 << r, dtype=np.int64)
+            full = pdep_array(e, list(d.pos), r)
+            if d.kind == _lib.TO_LEAF_VAR:
+                val = np.where((full == (1 << r) - 1) & (r > 0), d.val1,
+                               np.where((full == 0) | (r == 0), d.val0, 0.0))
+            else:
+                lit = full ^ d.negmask
+                if d.out_bit < 0:
+                    one = lit != 0
+                else:
+                    ob = 1 << d.out_bit
+                    one = ((lit & ob) != 0) == ((lit & ~ob) != 0)
+                val = np.where(one, d.val1, d.val0)
+            self.heap.view(d.dst, self.dt, 1 << r)[:] = val
+
     def op_7(self, L):
         params = self.heap.view(L.aux3, np.int64, 2)
         params[0] += L.iaux0
@@ -268,6 +288,7 @@ def emulate_plan(tree, tables, num_groups, total, entry_type, W, regroup=True):
     heap.write(prog.params, np.array([0, total, 0, 1], dtype=np.int64).tobytes())
     heap.write(prog.accum, bytes(_lib.Accum()))
     emu = Emulator(heap, entry_type)
+    emu.run(prog.build)
     emu.run(prog.prologue)
     if not prog.root_const:
         for _ in range(-(-total // W)):
@@ -277,15 +298,23 @@ def emulate_plan(tree, tables, num_groups, total, entry_type, W, regroup=True):
     return math.ldexp(acc.mant * fm, acc.exp + fe), prog
 
 
-def plan_tables(plan, api_like):
-    """(tree, tables, num_groups, total) exactly as HipAPI.contract_sliced builds them."""
+def plan_tables(plan, api_like, device_leaves=True):
+    """(tree, tables, num_groups, total) exactly as HipAPI.contract_sliced builds them (OR
+    and variable leaves described for the device to write, with ``device_leaves``)."""
     from tensororder_amd import schedule
+    from tensororder_amd.hip_api import leaf_spec_table
 
     groups = [g for g in plan.groups_to_slice if len(g) > 0]
     total, sequences, _ = plan.network.get_tensor_slices(plan.groups_to_slice, api_like.create_tensor)
     tree = plan.network.remove_sliced_indices_from(plan.tree, plan.groups_to_slice)
+    _total, lookups = plan.network.slice_lookups(plan.groups_to_slice)
+    assert _total == total
     tables = {}
     for t, seq in enumerate(sequences):
+        spec_table = leaf_spec_table(plan.network[t], lookups[t], len(groups)) if device_leaves else None
+        if spec_table is not None:
+            tables[t] = spec_table
+            continue
         reordered, lookup, shape = seq.reordered_tensor(api_like)
         reordered = np.asarray(reordered, dtype=np.float64)
         tables[t] = schedule.LeafTable(reordered.reshape(reordered.shape[0], -1), lookup, len(shape))
@@ -333,6 +362,8 @@ def emulate_batch(instances, entry_type, budget=None):
         if entry_type == _lib.TO_F32S:
             heap.write(prog.recs, batch.rec_init_bytes(prog))
     emu = Emulator(heap, entry_type)
+    for prog in progs:
+        emu.run(prog.build)
     for prog in progs:
         emu.run(prog.launches)
         for i in range(prog.count):

@@ -337,3 +337,37 @@ def test_batch_admission_waves_emulated(et, tol):
     assert len(progs) == len(waves)
     for g, w in zip(got, wants):
         assert g == pytest.approx(w, rel=tol)
+
+
+@pytest.mark.parametrize("et", [_lib.TO_F64, _lib.TO_F32S])
+def test_device_leaf_build_matches_host_tables(et):
+    # every OR / variable leaf kind under several slicings: the table the leaf build op
+    # writes (emulated from its documented semantics) equals the host's reordered table,
+    # or its float32 encoding
+    import leafcases
+
+    from tensororder_amd.hip_api import leaf_spec_table
+
+    scaled = et == _lib.TO_F32S
+    dt = np.float32 if scaled else np.float64
+    n = 0
+    for name, t, lookup, ngroups, rows, coef in leafcases.cases():
+        tab = leaf_spec_table(t, lookup, ngroups)
+        assert tab is not None and tab.coef == list(coef) and tab.nrows == rows.shape[0], name
+        heap = emulator.Heap(1 << 20)
+        buf = heap.alloc(tab.size * 8)
+        v0, v1, F, e = tab.spec.stored(scaled)
+        bd = tab.spec.desc(buf.data_ptr(), v0, v1)
+        blob = heap.alloc(ctypes.sizeof(bd))
+        heap.write(blob, bytes(bd))
+        emu = emulator.Emulator(heap, et)
+        emu.run([_lib.Launch(op=_lib.TO_OP_LEAF_BUILD, count=1, grid=bd.nblocks, descs=blob.data_ptr())])
+        got = heap.view(buf.data_ptr(), dt, tab.size).reshape(rows.shape)
+        if scaled:
+            want, F2, e2 = schedule.leaf_encoding(rows)
+            assert (F, e) == (F2, e2), name
+            np.testing.assert_array_equal(got, want.astype(np.float32), err_msg=name)
+        else:
+            np.testing.assert_array_equal(got, rows, err_msg=name)
+        n += 1
+    assert n > 100

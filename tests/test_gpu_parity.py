@@ -472,3 +472,70 @@ def test_batch_admission_waves(hip32, hip):
                 assert all(l >= 0 for l in lat)
         finally:
             exe.close()
+
+
+@pytest.mark.parametrize("et", ["f64", "f32s"])
+def test_device_leaf_tables_match_host_build(hip, hip32, et):
+    # every OR / variable leaf kind (plain and output-indexed OR, factored OR and variable
+    # pieces, rank 0..5, weights unweighted / cachet / mcc) under several slicings: the
+    # tables the leaf build kernel writes equal, byte for byte, the host's network build
+    # reordered as tensor.py:95-119 reorders it (float32 encoding in the scaled executor)
+    import ctypes
+
+    import leafcases
+    import torch
+
+    from tensororder_amd import _lib, schedule
+    from tensororder_amd.hip_api import leaf_spec_table
+
+    api = hip if et == "f64" else hip32
+    scaled = et == "f32s"
+    dt = np.float32 if scaled else np.float64
+    cases = leafcases.cases()
+    _say("case device leaves", et, len(cases))
+    offs, off, descs, wants, block0 = [], 0, [], [], 0
+    for name, t, lookup, ngroups, rows, coef in cases:
+        tab = leaf_spec_table(t, lookup, ngroups)
+        v0, v1, F, e = tab.spec.stored(scaled)
+        want = schedule.leaf_encoding(rows)[0].astype(np.float32) if scaled else rows
+        wants.append((name, want))
+        offs.append(off)
+        descs.append((tab.spec, v0, v1))
+        off += (tab.size + 63) // 64 * 64
+    esize = 4 if scaled else 8
+    buf = torch.full((off,), float("nan"), dtype=torch.float32 if scaled else torch.float64, device="cuda")
+    structs = []
+    for (spec, v0, v1), o in zip(descs, offs):
+        bd = spec.desc(buf.data_ptr() + o * esize, v0, v1)
+        bd.block0 = block0
+        block0 += bd.nblocks
+        structs.append(bd)
+    blob = torch.frombuffer(bytearray(b"".join(bytes(s) for s in structs)), dtype=torch.uint8).cuda()
+    torch.cuda.synchronize()
+    L = _lib.Launch(op=_lib.TO_OP_LEAF_BUILD, count=len(structs), grid=block0, descs=blob.data_ptr())
+    arr = _lib.launch_array([L])
+    _lib.check(api._lib.to_run(ctypes.addressof(arr), 1, api._entry_type, ctypes.c_void_p(api._stream.cuda_stream)),
+               "to_run")
+    api._stream.synchronize()
+    host = buf.cpu().numpy()
+    for (name, want), o in zip(wants, offs):
+        got = host[o:o + want.size].reshape(want.shape)
+        assert got.tobytes() == np.ascontiguousarray(want, dtype=dt).tobytes(), name
+
+
+def test_device_and_host_leaves_agree(hip, hip32):
+    # the same sliced plans with leaves written on the device and built on the host give
+    # the same counts, bit for bit
+    for name in ("cnf20_factor_s4", "vc22_weighted_line_s3"):
+        doc = json.load(open(os.path.join(HERE, "golden", name + ".json")))
+        for api in (hip, hip32):
+            _say("case device vs host leaves", name, api.get_entry_size())
+            plan = plan_io.plan_from_dict(doc)
+            api.add_argument("device_leaves", False)
+            try:
+                host = api.contract_sliced(plan)
+            finally:
+                api.add_argument("device_leaves", True)
+            dev = api.contract_sliced(plan)
+            assert api.last_stats["device_built_leaves"] == len(doc["tensors"])
+            assert dev == host

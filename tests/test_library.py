@@ -30,11 +30,11 @@ def test_exports_every_declared_symbol():
 def test_layouts_and_version():
     lib = _lib.load()
     assert lib.to_version() == _lib.ABI_VERSION
-    sizes = [ctypes.c_int64() for _ in range(8)]
+    sizes = [ctypes.c_int64() for _ in range(9)]
     lib.to_struct_sizes(*[ctypes.byref(s) for s in sizes])
     assert [s.value for s in sizes] == [ctypes.sizeof(t) for t in (
         _lib.JoinDesc, _lib.LeafDesc, _lib.ScaleRec, _lib.Accum, _lib.Launch, _lib.PermDesc, _lib.GemmDesc,
-        _lib.RootDesc)]
+        _lib.RootDesc, _lib.LeafBuildDesc)]
 
 
 def test_invalid_records_are_rejected_without_launching():
