@@ -363,6 +363,42 @@ def learned_choice(api, cand_path):
     return plans[choice], plans[ref], info
 
 
+def learned_eval(args):
+    """Every candidate plan of the eight held-out formulas (tests/golden/heldout_candidates.json)
+    timed in full with the FP32-scaled executor: per formula the learned pick, the
+    reference rule's pick, the FLOP rule's pick and the measured best; one JSON line with
+    the per-formula results and their medians (speed-up over the reference rule, over the
+    FLOP rule, and regret against the measured best)."""
+    import numpy as np
+
+    from tensororder_amd import costmodel, train_costmodel
+    from tensororder_amd.hip_api import HipF32API
+
+    with open(os.path.join(ROOT, "tests", "golden", "heldout_candidates.json")) as f:
+        sets = json.load(f)["sets"]
+    api = HipF32API()
+    model = costmodel.load_default(api)
+    rows = []
+    for doc in sets:
+        t0 = time.perf_counter()
+        r = train_costmodel.evaluate_choices(api, model, doc, measure_all=True)
+        m = r["measured_s"]
+        r["speedup_vs_reference"] = m["reference"] / m["learned"]
+        r["speedup_vs_flops"] = m["flops"] / m["learned"]
+        r["regret_vs_best"] = m["learned"] / r["best_s"] - 1.0
+        r["reference_regret_vs_best"] = m["reference"] / r["best_s"] - 1.0
+        rows.append(r)
+        log("%s: %d candidates in %.0f s; learned #%d %.3f s, reference #%d %.3f s, flops #%d %.3f s, best #%d %.3f s"
+            % (r["name"], r["candidates"], time.perf_counter() - t0, r["picks"]["learned"], m["learned"],
+               r["picks"]["reference"], m["reference"], r["picks"]["flops"], m["flops"], r["best"], r["best_s"]))
+    summary = {k: float(np.median([r[k] for r in rows]))
+               for k in ("speedup_vs_reference", "speedup_vs_flops", "regret_vs_best", "reference_regret_vs_best")}
+    summary["learned_faster_than_reference"] = sum(r["speedup_vs_reference"] > 1 for r in rows)
+    summary["learned_is_best"] = sum(r["picks"]["learned"] == r["best"] for r in rows)
+    print(json.dumps({"learned_eval": rows, "median": summary, "formulas": len(rows),
+                      "executor": "fp32-scaled", "host": host_description()}), flush=True)
+
+
 def batch_bench(args):
     """The mixed batch: every instance of the batch fixture contracted by one grouped
     schedule (one HIP graph launch per step); value = instances/s of the FP32-scaled
@@ -465,6 +501,8 @@ def main():
                     help="skip the learned plan choice where the workload has candidate plans")
     ap.add_argument("--permute-sweep", action="store_true",
                     help="only time standalone bit permutations of rank-26 tensors")
+    ap.add_argument("--learned-eval", action="store_true",
+                    help="only time every candidate plan of the held-out formulas (learned vs reference vs FLOP rule)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 512 MiB three times (reference for HBM counters)")
     args = ap.parse_args()
@@ -473,6 +511,9 @@ def main():
         return
     if args.permute_sweep:
         permute_sweep()
+        return
+    if args.learned_eval:
+        learned_eval(args)
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the executor has no multi-GPU path in this build "

@@ -250,6 +250,16 @@ def reference_choice(plans):
     return best
 
 
+def flop_choice(plans):
+    """A FLOP rule: the candidate of smallest estimated FLOPs (the reference's cost model,
+    contraction_tree.pyx:449-458, times the slice count), earliest on ties."""
+    best = 0
+    for i, p in enumerate(plans):
+        if p.total_FLOPs < plans[best].total_FLOPs:
+            best = i
+    return best
+
+
 def learned_choice(model, api, plans):
     """Index of the candidate with the smallest predicted device time, and all predictions.
     A candidate that does not fit the device even one slice at a time is never chosen."""

@@ -237,8 +237,8 @@ SLICE_SPECS = (
 
 def plan_candidates(network, planner, decompositions, slices, specs=SLICE_SPECS, seed=0):
     """Candidate plans of one network: every decomposition (TreeDecomposition) x every
-    slice spec, each sliced to exactly ``slices`` groups (a spec that needs more groups to
-    meet its own limits is dropped).  Returns (plans, labels); a label names the
+    slice spec, each sliced to exactly ``slices`` distinct groups (a spec that needs more
+    groups to meet its own limits, or that slices one group twice, is dropped).  Returns (plans, labels); a label names the
     decomposition index, its unsliced max rank and the slice spec."""
     import random as _random
 
@@ -262,6 +262,12 @@ def plan_candidates(network, planner, decompositions, slices, specs=SLICE_SPECS,
             finally:
                 _random.setstate(state)
             if len(plan.groups_to_slice) != slices:
+                continue
+            # greedy_largest may pick an edge that is already sliced (slicers.py:36-49
+            # chooses among all free edges of the largest node) and slice its group again:
+            # the first copy then fixes no leaf and the sum over it doubles the count (the
+            # reference's own execution does the same).  Such a plan is no candidate.
+            if len({frozenset(g) for g in plan.groups_to_slice}) != len(plan.groups_to_slice):
                 continue
             plans.append(plan)
             labels.append({"decomposition": di, "width": width, "unsliced_maxrank": unsliced,

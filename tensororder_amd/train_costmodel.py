@@ -96,14 +96,44 @@ def training_plans(count, seed0=0):
 
 
 def load_candidates(path):
-    """(plans, labels, reference index, DIMACS text) of a candidate fixture."""
-    with open(path) as f:
-        doc = json.load(f)
+    """(plans, labels, reference index, document) of a candidate fixture (a path, or one
+    candidate document already loaded)."""
+    if isinstance(path, dict):
+        doc = path
+    else:
+        with open(path) as f:
+            doc = json.load(f)
     net = network.cnf_count(parse_dimacs_text(doc["dimacs"], doc["weights"]))
     tds = [decomposition.parse_all(t["text"])[0] for t in doc["decompositions"]]
     planner = "line" if doc["planner"].startswith("line") else "factor"
     plans, labels = planners.plan_candidates(net, planner, tds, doc["slices"])
     return plans, labels, planners.reference_candidate(labels), doc
+
+
+def evaluate_choices(api, model, doc, measure_all=False, reps=1):
+    """The plan rules on one candidate set: the learned pick (smallest predicted device
+    time), the reference rule's (planning.py:159-162 with greedy_mem slices) and the FLOP
+    rule's (smallest estimated FLOPs), each timed in full on ``api`` (and, with
+    ``measure_all``, every candidate, which names the measured best).  Returns a dict."""
+    from . import costmodel
+
+    plans, labels, ref, doc = load_candidates(doc)
+    learned, preds = costmodel.learned_choice(model, api, plans)
+    flop = costmodel.flop_choice(plans)
+    picks = {"learned": learned, "reference": ref, "flops": flop}
+    times, counts = {}, {}
+    todo = range(len(plans)) if measure_all else sorted(set(picks.values()))
+    for i in todo:
+        times[i], counts[i] = measure(api, plans[i], reps=reps)
+    out = {"name": doc.get("name"), "candidates": len(plans), "slices": doc["slices"],
+           "picks": picks, "labels": {k: labels[v] for k, v in picks.items()},
+           "predicted_s": {k: preds[v] for k, v in picks.items()},
+           "measured_s": {k: times[v] for k, v in picks.items()}, "counts": {k: counts[v] for k, v in picks.items()}}
+    if measure_all:
+        best = min(times, key=times.get)
+        out.update(best=best, best_label=labels[best], best_s=times[best],
+                   all_measured_s=[times[i] for i in range(len(plans))], all_predicted_s=list(preds))
+    return out
 
 
 def measure(api, plan, reps=1, warmup=1):

@@ -7,7 +7,7 @@ The fixture stores the DIMACS instance and those decompositions (PACE .td text);
 rebuilt from them deterministically by ``planners.plan_candidates`` (decompositions x
 slice specs, 12 slice groups each).
 
-    python tests/golden/make_candidates.py [seconds]
+    python tests/golden/make_candidates.py [seconds] [--relabel]
 """
 
 import json
@@ -26,13 +26,19 @@ SEEDS = (0, 2, 3, 7)
 
 
 def main():
-    seconds = float(sys.argv[1]) if len(sys.argv) > 1 else 15.0
+    relabel = "--relabel" in sys.argv  # keep the stored decompositions, rebuild the labels
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    seconds = float(args[0]) if args else 15.0
     with open(os.path.join(HERE, "cnf70_factor_s12.json")) as f:
         dimacs = json.load(f)["meta"]["dimacs"]
     net = network.cnf_count(parse_dimacs_text(dimacs, "cachet"))
     text = net.structure_text()
     tds = []
-    for seed in SEEDS:
+    if relabel:
+        with open(os.path.join(HERE, "cnf70_candidates.json")) as f:
+            old = json.load(f)
+        tds, seconds = old["decompositions"], float(old["solver"].split("after ")[1].split(" s")[0])
+    for seed in SEEDS if not relabel else ():
         got = decomposition.run_solver([FLOWCUTTER, "-s", str(seed), "-p", "100"], text,
                                        max_decompositions=100000, time_limit=seconds)
         td = got[-1]

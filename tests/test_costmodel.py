@@ -134,3 +134,103 @@ def test_trained_model_choice_beats_reference_rule():
     print("measured: reference %.3f s, learned %.3f s" % (t_ref, t_new), flush=True)
     assert abs(v_new - v_ref) <= 1e-6 * abs(v_ref)
     assert t_new < t_ref
+
+
+HELDOUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "heldout_candidates.json")
+
+
+def _heldout_sets():
+    import json
+
+    with open(HELDOUT) as f:
+        return json.load(f)["sets"]
+
+
+def test_heldout_candidate_sets_rebuild():
+    # eight formulas no training run saw (60-67 variables, seeds 10000+): each candidate
+    # set rebuilds to its stored labels, every candidate sliced to the set's group count
+    from tensororder_amd import train_costmodel
+
+    sets = _heldout_sets()
+    assert len(sets) == 8
+    for doc in sets[:2]:
+        plans, labels, ref, _ = train_costmodel.load_candidates(doc)
+        assert labels == doc["candidates"] and ref == doc["reference"] and len(plans) >= 16
+        assert all(len(p.groups_to_slice) == doc["slices"] for p in plans)
+        assert costmodel.flop_choice(plans) in range(len(plans))
+
+
+_HELDOUT_RESULTS = {}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("i", range(8))
+def test_heldout_learned_choice(i):
+    # one held-out formula: the learned pick, the reference rule's and the FLOP rule's run
+    # in full with the FP32-scaled executor; every pick is an exact plan of the same
+    # network, so the counts agree
+    from tensororder_amd import train_costmodel
+    from tensororder_amd.hip_api import HipF32API
+
+    api = HipF32API()
+    model = costmodel.load_default(api)
+    r = train_costmodel.evaluate_choices(api, model, _heldout_sets()[i])
+    print("held-out", r["name"], "measured", r["measured_s"], "picks", r["picks"], flush=True)
+    vals = list(r["counts"].values())
+    assert max(abs(v - vals[0]) for v in vals) <= 1e-6 * abs(vals[0])
+    _HELDOUT_RESULTS[i] = r
+
+
+@pytest.mark.gpu
+def test_heldout_learned_beats_reference_on_median():
+    # over the eight held-out formulas the learned pick runs faster than the reference
+    # rule's on the median (speed-up > 1)
+    from tensororder_amd import train_costmodel
+    from tensororder_amd.hip_api import HipF32API
+
+    api = None
+    for i in range(8):
+        if i not in _HELDOUT_RESULTS:
+            api = api or HipF32API()
+            _HELDOUT_RESULTS[i] = train_costmodel.evaluate_choices(api, costmodel.load_default(api),
+                                                                    _heldout_sets()[i])
+    gains = [r["measured_s"]["reference"] / r["measured_s"]["learned"] for r in _HELDOUT_RESULTS.values()]
+    print("learned vs reference speed-ups", ["%.2f" % g for g in gains], flush=True)
+    assert float(np.median(gains)) > 1.0
+
+
+@pytest.mark.gpu
+def test_autotune_tune_plan():
+    # the tuner times every product and permute shape of a plan missing from a cache; the
+    # plan then runs with the tuned shapes and still matches the oracle
+    import json
+
+    from oracle import numpy_oracle
+    from tensororder_amd import autotune, plan_io
+    from tensororder_amd.hip_api import HipF32API
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "vc90_weighted_line.json")
+    doc = json.load(open(path))
+    want = doc["meta"]["expected"]["float64"]
+    from test_emulator import _two_tensor_plan
+
+    api = HipF32API()
+    for plan in (_two_tensor_plan(16, 15, 8, seed=3), plan_io.plan_from_dict(doc)):
+        total, tree, tables, groups = api._plan_inputs(plan, None)
+        nodes = schedule.analyse(tree, tables, True)
+        W = api.choose_batch(nodes, tables, total)
+        saved = dict(schedule.TUNED)
+        cache = {}
+        try:
+            tuned = autotune.tune_plan(api, nodes, W, cache)
+            for key, val in cache.items():
+                assert key.startswith(("gemm/", "perm/")) and val.get("ms_per_item", 0) > 0
+            schedule.TUNED.clear()
+            schedule.TUNED.update(cache)
+            got = api.contract_sliced(plan)
+        finally:
+            schedule.TUNED.clear()
+            schedule.TUNED.update(saved)
+        ref = numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan)) if plan is not None and tuned >= 0 else want
+        assert got == pytest.approx(ref, rel=1e-6)
+    assert tuned >= 0

@@ -273,3 +273,19 @@ def test_early_contraction_preserves_the_count(early):
     assert is_tree_complete(plan.tree, plan.network)
     got = numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan))
     assert got == pytest.approx(want, rel=1e-12)
+
+
+def test_candidates_never_slice_one_group_twice():
+    # greedy_largest can choose an edge whose group is already sliced; slicing it again
+    # leaves the first copy fixing no leaf, and the sum over it doubles the count -- the
+    # oracle (like the reference's own execution) shows it; such plans are no candidates
+    f = parse_dimacs_text(random_3cnf(22, 44, 5), "cachet")
+    net = network.cnf_count(f)
+    tds = planners.decompose(net.structure_text(), None, tie_seeds=[0, 1])
+    plans, labels = planners.plan_candidates(net, "factor", tds, 3)
+    for p in plans:
+        assert len({frozenset(g) for g in p.groups_to_slice}) == len(p.groups_to_slice)
+    doc = plan_io.plan_to_dict(plans[0])
+    want = numpy_oracle.contract_sliced(doc)
+    doc["groups_to_slice"] = doc["groups_to_slice"] + [doc["groups_to_slice"][0]]
+    assert numpy_oracle.contract_sliced(doc) == pytest.approx(2 * want, rel=1e-12)
