@@ -59,7 +59,7 @@ def kernel_symbol(L, scaled):
     if op == _lib.TO_OP_GEMM:
         wide, tr = "true" if v & 2 else "false", "true" if v & 4 else "false"
         if v & 16:
-            return "k_gemm_x6<%s, %s, %s>" % (S, wide, tr)
+            return "k_gemm_x6<%s, %s, %s, %d>" % (S, wide, tr, 256 if v & 32 else 128)
         if v & 8:
             return "k_gemm128_db<%s, %s, %s, %s>" % (T, S, wide, tr)
         if v & 1:
