@@ -853,7 +853,8 @@ __device__ __forceinline__ void join_stream_vec_body(const to_join_desc* __restr
 #pragma unroll
                 for (int r = 0; r < E / (64 * VF); ++r) {
                     const int e = (r * 64 + lane) * VF;
-                    *reinterpret_cast<TO_GLOBAL V*>(Cw + q * E + e) = *reinterpret_cast<const V*>(st + e);
+                    __builtin_nontemporal_store(*reinterpret_cast<const V*>(st + e),
+                                                reinterpret_cast<TO_GLOBAL V*>(Cw + q * E + e));
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 __builtin_amdgcn_wave_barrier();
@@ -870,7 +871,7 @@ __device__ __forceinline__ void join_stream_vec_body(const to_join_desc* __restr
                     o[c] = v;
                     if (SCALED) m = fmaxf(m, (float)absval(v));
                 }
-                *reinterpret_cast<TO_GLOBAL V*>(Cp + j) = o;
+                __builtin_nontemporal_store(o, reinterpret_cast<TO_GLOBAL V*>(Cp + j));
             }
         }
         if (SCALED) {
