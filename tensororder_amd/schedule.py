@@ -19,7 +19,6 @@ The reference walks the tree once per slice, calling tensordot on every join
 
 import ctypes
 import math
-import os
 
 import numpy as np
 
@@ -211,35 +210,36 @@ def join_geometry(lrank, rrank, lmap, rmap):
                 xf=b_free, xs=b_sh, yf=a_free, ys=a_sh)
 
 
+# Kernel-choice constants (settled by measurement on MI355X; each alternative they rule out
+# is recorded beside the kernel it concerns).
 # largest GEMM output tile edge to use (64 or 128)
-GEMM_TILES = int(os.environ.get("TENSORORDER_GEMM_TILES", "128"))
+GEMM_TILES = 128
 # stage streamed-join outputs through LDS when they form one contiguous run per tile
-STAGE_OUTPUTS = os.environ.get("TENSORORDER_STAGE_OUTPUTS", "1") != "0"
-# the MFMA path for dense joins (TENSORORDER_GEMM=0 turns it off)
-USE_GEMM = os.environ.get("TENSORORDER_GEMM", "1") != "0"
-# the 16-byte streaming join form (TENSORORDER_STREAM_VEC=0 turns it off)
-STREAM_VEC = os.environ.get("TENSORORDER_STREAM_VEC", "1") != "0"
+STAGE_OUTPUTS = True
+# the MFMA path for dense joins
+USE_GEMM = True
+# the 16-byte streaming join form
+STREAM_VEC = True
 # streamed joins of one level moving fewer bytes than this share one launch of the
-# mixed-form kernel (TENSORORDER_STREAM_MERGE_BYTES=0 turns it off)
-STREAM_MERGE_BYTES = int(os.environ.get("TENSORORDER_STREAM_MERGE_BYTES", str(4 << 20)))
-# the row form of streamed joins with long contiguous output rows (TENSORORDER_STREAM_ROWS=0)
-STREAM_ROWS = os.environ.get("TENSORORDER_STREAM_ROWS", "1") != "0"
+# mixed-form kernel
+STREAM_MERGE_BYTES = 4 << 20
+# the row form of streamed joins with long contiguous output rows
+STREAM_ROWS = True
 # float GEMMs with at least this many shared bits accumulate partial sums in double
-WIDE_ACCUM_MIN_SHARED = int(os.environ.get("TENSORORDER_WIDE_ACCUM_MIN_SHARED", "11"))
+WIDE_ACCUM_MIN_SHARED = 11
 # split-K operands with at least this many shared bits are permuted first when their
-# lowest shared bits are scattered (TENSORORDER_SPLITK_PERMUTE_MIN_SHARED)
-SPLITK_PERMUTE_MIN_SHARED = int(os.environ.get("TENSORORDER_SPLITK_PERMUTE_MIN_SHARED", "14"))
+# lowest shared bits are scattered
+SPLITK_PERMUTE_MIN_SHARED = 14
 GEMM_MIN_FREE = 6
 GEMM_MIN_SHARED = 4
 GEMM128_MIN_SHARED = {(8, False): 4, (4, False): 5, (4, True): 5}
 MAX_GEMM_BLOCKS = 8192
 # by rule, wide float products stay on 128x64 tiles at two blocks per CU (measured faster
-# on MI355X than 128x128 at one block per CU); TENSORORDER_GEMM_WIDE_DB=1 picks the latter
-# (the autotuner chooses per shape either way)
-GEMM_WIDE_DB = os.environ.get("TENSORORDER_GEMM_WIDE_DB", "0") != "0"
-# float products with 128x128 tiles on the bf16 matrix cores, each operand split into three
-# bf16 parts (TENSORORDER_GEMM_X6=0 keeps them on the float MFMA)
-GEMM_X6 = os.environ.get("TENSORORDER_GEMM_X6", "1") != "0"
+# on MI355X than 128x128 at one block per CU); the autotuner chooses per shape either way
+GEMM_WIDE_DB = False
+# float products with 128x128 (or 256x128) tiles on the bf16 matrix cores, each operand
+# split into three bf16 parts
+GEMM_X6 = True
 
 
 def _canonical(free_pos, shared_pos):
@@ -683,8 +683,8 @@ def plan_memory(nodes, tables, W, esize, order="post"):
 
 
 # level-grouped execution is used when its arena peak stays within this factor of the
-# post-order peak (TENSORORDER_LEVEL_GROUPS=0 turns it off)
-LEVEL_GROUPS = os.environ.get("TENSORORDER_LEVEL_GROUPS", "1") != "0"
+# post-order peak
+LEVEL_GROUPS = True
 LEVEL_PEAK_FACTOR = 1.10
 
 
