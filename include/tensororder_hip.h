@@ -57,6 +57,10 @@ extern "C" {
 #define TO_STREAM_ROWS 0x200
 #define TO_STREAM_ANY 0x400  /* one launch of streamed joins of mixed variants, each given by
                                 its descriptor's kvar; lds = the largest any of them needs */
+#define TO_STREAM_WAVE 0x800 /* one wave (64 lanes) per join, for small joins (s <= 6, ranks
+                                of X, Y and C below 32): descriptor i is run by wave i % 4
+                                of block i / 4 (grid >= count / 4, rounded up; block0 /
+                                nblocks / kvar unread), no LDS */
 #define TO_OP_JOIN_SPLITK 3  /* descs: to_join_desc[count]; partial sums (double) to d.partial */
 #define TO_OP_JOIN_FINISH 4  /* descs: to_join_desc[count]; sums d.partial into C; variant 1:
                                 one block per output (for nsplit >= 32) */
@@ -215,7 +219,12 @@ typedef struct to_launch {
     int32_t lds;    /* dynamic LDS bytes */
     int32_t flags;  /* TO_LAUNCH_* (graph capture only; to_run runs records in order) */
     uint64_t descs; /* device pointer to the descriptor array */
-    uint64_t aux0, aux1, aux2, aux3;
+    uint64_t aux0, aux1, aux2, aux3; /* per op; for the descriptor-table ops (LEAF_GATHER,
+                                        JOIN_STREAM, JOIN_SPLITK, JOIN_FINISH, PERMUTE, GEMM,
+                                        LEAF_BUILD) aux1 is 0 or a device int32[grid] map of
+                                        each block to its descriptor (the last one whose
+                                        block0 is at or below the block), read instead of a
+                                        binary search over block0 */
     int64_t iaux0, iaux1;
 } to_launch;
 

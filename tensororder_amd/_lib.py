@@ -31,6 +31,7 @@ TO_LEAF_VAR, TO_LEAF_OR = 0, 1  # leaf build kinds
 TO_STREAM_VEC = 0x100  # stream variant bit: the 16-byte form
 TO_STREAM_ROWS = 0x200  # stream variant: the row form
 TO_STREAM_ANY = 0x400  # stream launch variant: mixed joins, each by its descriptor's kvar
+TO_STREAM_WAVE = 0x800  # stream launch variant: one wave per (small) join, four per block
 TO_LAUNCH_SIDE = 0x1  # Launch.flags: captured on the side stream
 TO_LAUNCH_JOIN = 0x2  # Launch.flags: join the side stream into the main one and fork it again first
 MAX_BITS = 32

@@ -3,8 +3,9 @@
 Each instance keeps its own compiled join list and its own arena region (placed for level
 order).  Step k of the batch runs every join of tree level k of every instance (a join's
 level is one more than its children's; joins of one level are independent), as grouped
-launches (one per kernel variant) whose blocks locate their join by binary search over
-``block0`` -- so a batch of N instances needs max_i(height_i) steps rather than
+launches (one per kernel variant) whose blocks locate their join through a block map
+(or by binary search over ``block0``), and whose smallest streamed joins run one wave
+each, four to a block -- so a batch of N instances needs max_i(height_i) steps rather than
 sum_i(#joins_i) launches, and independent small joins share the machine.  Right after its
 last level an instance's root is stored into its own accumulator together with the device
 clock, which gives per-instance completion times within the batch.
@@ -19,6 +20,9 @@ from ._lib import Launch
 
 ALIGN = schedule.ALIGN
 KIND_ORDER = schedule.KIND_ORDER
+# a grouped launch over at least this many descriptors also carries a block -> descriptor
+# map (int32 per block, in aux1), so each block finds its descriptor in one load
+MAP_MIN_DESCS = 8
 
 
 class BatchProgram:
@@ -176,10 +180,10 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True, per=None, a
             k = step_of[n.idx]
             side = n.geom["var"]["kind"] in ("stream", "splitk")
             for kind, op, variant, lds, st, nblk in schedule.join_records(
-                    n, nodes, 1, esize, scaled, abase, scratch.get(n.idx, {}), recp):
+                    n, nodes, 1, esize, scaled, abase, scratch.get(n.idx, {}), recp, launch_bound=True):
                 # the streamed joins of one step share one launch whatever their form
                 # (each descriptor names its own); the step is bound by launch count
-                if op == _lib.TO_OP_JOIN_STREAM:
+                if op == _lib.TO_OP_JOIN_STREAM and variant != _lib.TO_STREAM_WAVE:
                     schedule.check_stream_form(st.kvar)
                     variant = _lib.TO_STREAM_ANY
                 key = (side, KIND_ORDER[kind], op, variant)
@@ -196,13 +200,15 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True, per=None, a
 
     blob = bytearray()
 
-    def put(structs):
+    def put_bytes(data):
         nonlocal blob
         start = (len(blob) + ALIGN - 1) // ALIGN * ALIGN
         blob += b"\0" * (start - len(blob))
-        for st in structs:
-            blob += bytes(st)
+        blob += data
         return start
+
+    def put(structs):
+        return put_bytes(b"".join(bytes(st) for st in structs))
 
     # leaf tables written on the device once, right after upload
     build_off, build_grid = None, 0
@@ -217,11 +223,11 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True, per=None, a
 
     pending = []
     pending.append((Launch(op=_lib.TO_OP_BATCH_BEGIN, grid=max(1, min(64, -(-len(per) // 256))),
-                           aux2=prog.accum.data_ptr(), aux3=prog.start.data_ptr(), iaux0=len(per)), None,
+                           aux2=prog.accum.data_ptr(), aux3=prog.start.data_ptr(), iaux0=len(per)), (None, None),
                     ("begin", -1), (0, 0)))
     if scaled and nrec > join_rec0:
         pending.append((Launch(op=_lib.TO_OP_SCALE_CLEAR, grid=max(1, min(1024, -(-(nrec - join_rec0) // 256))),
-                               aux0=rbase + join_rec0 * rec_size, iaux0=nrec - join_rec0), None, ("clear", -1),
+                               aux0=rbase + join_rec0 * rec_size, iaux0=nrec - join_rec0), (None, None), ("clear", -1),
                         (0, 0)))
     nsteps = max(list(steps) + list(roots)) + 1 if (steps or roots) else 0
     launches_per_kind = {}
@@ -230,23 +236,34 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True, per=None, a
         for key in sorted(steps.get(k, {})):
             group = steps[k][key]
             structs, block0, lds, wb, wf = [], 0, 0, 0, 0
-            for st, nblk, l, kind, i, work in group:
-                st.block0 = block0
-                block0 += nblk
+            waves = key[2] == _lib.TO_OP_JOIN_STREAM and key[3] == _lib.TO_STREAM_WAVE
+            for j, (st, nblk, l, kind, i, work) in enumerate(group):
+                if waves:
+                    # descriptor j runs on wave j % 4 of block j / 4
+                    st.block0, st.nblocks = j // 4, 1
+                    block0 = j // 4 + 1
+                else:
+                    st.block0 = block0
+                    block0 += nblk
                 lds = max(lds, l)
                 structs.append(st)
                 wb += work[0]
                 wf += work[1]
             off = put(structs)
+            moff = None
+            if len(structs) >= MAP_MIN_DESCS and not waves:
+                # block -> descriptor map: one load per block instead of a binary search
+                nblks = [g[1] for g in group]
+                moff = put_bytes(np.repeat(np.arange(len(structs), dtype=np.int32), nblks).tobytes())
             kind = group[0][3]
             pending.append((Launch(op=key[2], variant=key[3], count=len(structs), grid=block0, lds=lds,
-                                   flags=_lib.TO_LAUNCH_SIDE if key[0] else 0), off, (kind, k), (wb, wf)))
+                                   flags=_lib.TO_LAUNCH_SIDE if key[0] else 0), (off, moff), (kind, k), (wb, wf)))
             launches_per_kind[kind] = launches_per_kind.get(kind, 0) + 1
         if k in roots:
             rs = roots[k]
             off = put(rs)
             pending.append((Launch(op=_lib.TO_OP_ROOT_STORE, count=len(rs), grid=max(1, -(-len(rs) // 256))),
-                            off, ("root", k), (0, 0)))
+                            (off, None), ("root", k), (0, 0)))
         if len(pending) > step_start:
             # the step starts from everything of the previous one, on both branches
             pending[step_start][0].flags |= _lib.TO_LAUNCH_JOIN
@@ -257,9 +274,11 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True, per=None, a
         Launch(op=_lib.TO_OP_LEAF_BUILD, count=len(builds), grid=build_grid, descs=bbase + build_off)]
     # per launch: (kind, step) and the algorithmic (bytes, flops) of its joins
     prog.work = []
-    for L, off, info, work in pending:
+    for L, (off, moff), info, work in pending:
         if off is not None:
             L.descs = bbase + off
+        if moff is not None:
+            L.aux1 = bbase + moff
         prog.launches.append(L)
         prog.info.append(info)
         prog.work.append(work)

@@ -68,10 +68,14 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// index of the descriptor whose block range holds block bid; the result is the same for
-// every lane of the block, and is marked wave-uniform so descriptor fields load as scalars
+// index of the descriptor whose block range holds block bid: read from the launch's
+// block -> descriptor map when it has one (one load), else by binary search over block0
+// (about log2(n) dependent loads); the result is the same for every lane of the block, and
+// is marked wave-uniform so descriptor fields load as scalars
 template <class D>
-__device__ __forceinline__ int find_desc(const D* __restrict__ descs, int n, int bid) {
+__device__ __forceinline__ int find_desc(const D* __restrict__ descs, int n, int bid,
+                                         const int32_t* __restrict__ bmap) {
+    if (bmap) return __builtin_amdgcn_readfirstlane(bmap[bid]);
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -157,8 +161,9 @@ using Acc = double;
 
 template <typename T, bool SCALED>
 __global__ __launch_bounds__(kBlock) void k_leaf_gather(const to_leaf_desc* __restrict__ descs,
-                                                        int ndesc, const int64_t* __restrict__ wave) {
-    const to_leaf_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+                                                        int ndesc, const int32_t* __restrict__ bmap,
+                                                        const int64_t* __restrict__ wave) {
+    const to_leaf_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     const int rank = d->rank;
     const int64_t E = 1ll << rank;
     const int64_t total = (int64_t)d->nbatch << rank;
@@ -187,8 +192,8 @@ __global__ __launch_bounds__(kBlock) void k_leaf_gather(const to_leaf_desc* __re
 // leaf build: OR and variable leaf tables written from their descriptions, once per plan
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_leaf_build(const to_leaf_build_desc* __restrict__ descs, int ndesc) {
-    const to_leaf_build_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+__global__ __launch_bounds__(kBlock) void k_leaf_build(const to_leaf_build_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
+    const to_leaf_build_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     const int rank = d->rank;
     const int64_t E = 1ll << rank;
     const int64_t all = E - 1;
@@ -438,9 +443,9 @@ __device__ __forceinline__ void join_stream_body(const to_join_desc* __restrict_
 }
 
 template <typename T, bool SCALED, int YC>
-__global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kBlock) void k_join_stream(const to_join_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     join_stream_body<T, SCALED, YC>(d, blockIdx.x - d->block0, smem);
 }
 
@@ -501,8 +506,8 @@ __device__ __forceinline__ void join_splitk_body(const to_join_desc* __restrict_
 }
 
 template <typename T, bool SCALED>
-__global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __restrict__ descs, int ndesc) {
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+__global__ __launch_bounds__(kBlock) void k_join_splitk(const to_join_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     join_splitk_body<T, SCALED>(d, blockIdx.x - d->block0);
 }
 
@@ -533,8 +538,8 @@ __device__ __forceinline__ void join_finish_body(const to_join_desc* __restrict_
 }
 
 template <typename T, bool SCALED>
-__global__ __launch_bounds__(kBlock) void k_join_finish(const to_join_desc* __restrict__ descs, int ndesc) {
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+__global__ __launch_bounds__(kBlock) void k_join_finish(const to_join_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     join_finish_body<T, SCALED>(d, blockIdx.x - d->block0);
 }
 
@@ -570,8 +575,8 @@ __device__ __forceinline__ void join_finish_wide_body(const to_join_desc* __rest
 }
 
 template <typename T, bool SCALED>
-__global__ __launch_bounds__(kBlock) void k_join_finish_wide(const to_join_desc* __restrict__ descs, int ndesc) {
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+__global__ __launch_bounds__(kBlock) void k_join_finish_wide(const to_join_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     join_finish_wide_body<T, SCALED>(d, blockIdx.x - d->block0);
 }
 
@@ -894,9 +899,9 @@ __device__ __forceinline__ void join_stream_vec_body(const to_join_desc* __restr
 }
 
 template <typename T, bool SCALED, int YC>
-__global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kBlock) void k_join_stream_vec(const to_join_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     join_stream_vec_body<T, SCALED, YC>(d, blockIdx.x - d->block0, smem);
 }
 
@@ -997,17 +1002,17 @@ __device__ __forceinline__ void join_rows_body(const to_join_desc* __restrict__ 
 }
 
 template <typename T, bool SCALED>
-__global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __restrict__ descs, int ndesc) {
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+__global__ __launch_bounds__(kBlock) void k_join_rows(const to_join_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     join_rows_body<T, SCALED>(d, blockIdx.x - d->block0);
 }
 
 // Streamed joins of mixed variants in one launch (the many-instance batch groups the joins
 // of one step this way): each block runs the form its descriptor's kvar names.
 template <typename T, bool SCALED>
-__global__ __launch_bounds__(kBlock) void k_join_stream_any(const to_join_desc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kBlock) void k_join_stream_any(const to_join_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const to_join_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     const int local = blockIdx.x - d->block0;
     switch (d->kvar) {
 #define TO_ANY_CASE(YC)                                                  \
@@ -1028,6 +1033,108 @@ __global__ __launch_bounds__(kBlock) void k_join_stream_any(const to_join_desc* 
             break;
         default:
             break;  // rejected on the host (schedule.stream_variant)
+    }
+}
+
+// Streamed joins small enough for one wave each (TO_STREAM_WAVE; the many-instance batch
+// has tens of thousands of them per step): descriptor i is run by wave i % 4 of block i / 4,
+// with no LDS and no barrier, so a block costs four joins one descriptor read each instead
+// of one join a table set-up.  The bit deposits are computed once per join, one per lane:
+// lane t holds the X / Y offset that bit t of the output index contributes, and lane k the
+// offsets of shared index k; the loops read them back with v_readlane (uniform index).  Lanes
+// walk the outputs of each batch item with output bits 0-5 on the lane id; bits 0.. of the
+// output index are y when shy == 0 (f otherwise).  Requires s <= 6 and ranks of X, Y and
+// the output below 32.
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_join_wave(const to_join_desc* __restrict__ descs, int ndesc) {
+    const int idx = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
+    if (idx >= ndesc) return;
+    const to_join_desc* d = descs + idx;
+    const int lane = threadIdx.x & 63;
+    const int fx = d->fx, fy = d->fy, s = d->s;
+    const int K = 1 << s;
+    const int nbits = fx + fy;
+    const int nout = 1 << nbits;
+    const bool yinner = d->shy == 0;
+    const int shf = d->shf, shy = d->shy;
+    // lane t < nbits: what output-index bit t adds to the X offset, the Y offset, and to the
+    // output's f and y (as a bit of f << shf | y << shy, which is the output offset)
+    uint32_t cx = 0, cy = 0, cc = 0;
+    if (lane < nbits) {
+        const bool ybit = yinner ? lane < fy : lane >= fx;
+        const int b = ybit ? (yinner ? lane : lane - fx) : (yinner ? lane - fy : lane);
+        if (ybit) {
+            cy = 1u << d->yf_pos[b];
+            cc = 1u << (b + shy);
+        } else {
+            cx = 1u << d->xf_pos[b];
+            cc = 1u << (b + shf);
+        }
+    }
+    // lane k < K: the offsets of shared index k in X and in Y
+    uint32_t kxl = 0, kyl = 0;
+    if (lane < K) {
+        kxl = (uint32_t)pdep(lane, d->xs_pos, s);
+        kyl = (uint32_t)pdep(lane, d->ys_pos, s);
+    }
+    // the offsets of this lane's own output bits 0-5
+    uint32_t xlane = 0, ylane = 0, clane = 0;
+    const int nlow = nbits < 6 ? nbits : 6;
+    for (int t = 0; t < nlow; ++t) {
+        const uint32_t a = lane_value(cx, t), b = lane_value(cy, t), c = lane_value(cc, t);
+        if ((lane >> t) & 1) {
+            xlane |= a;
+            ylane |= b;
+            clane |= c;
+        }
+    }
+    const TO_GLOBAL T* __restrict__ X = gptr<const T>(d->x);
+    const TO_GLOBAL T* __restrict__ Y = gptr<const T>(d->y);
+    TO_GLOBAL T* __restrict__ C = gptr<T>(d->c);
+    for (int64_t b = 0; b < d->nbatch; ++b) {
+        float sx = 1.0f, sy = 1.0f;
+        int64_t e = 0;
+        if (SCALED) {
+            const OpScale a = op_scale(d->xs, d->xsb, b);
+            const OpScale c = op_scale(d->ys, d->ysb, b);
+            sx = a.f; sy = c.f; e = a.e + c.e;
+        }
+        const auto Xb = X + b * d->xb;
+        const auto Yb = Y + b * d->yb;
+        const auto Cb = C + b * d->cb;
+        float m = 0.0f;
+        for (int j = 0; (j << 6) < nout; ++j) {
+            // output bits 6.. are those of j, the same in every lane
+            uint32_t xj = 0, yj = 0, cj = 0;
+            for (int t = 6; t < nbits; ++t) {
+                if ((j >> (t - 6)) & 1) {
+                    xj |= lane_value(cx, t);
+                    yj |= lane_value(cy, t);
+                    cj |= lane_value(cc, t);
+                }
+            }
+            if ((j << 6) + lane < nout) {
+                const auto Xp = Xb + (xlane | xj);
+                const auto Yp = Yb + (ylane | yj);
+                Acc<T> acc = 0.0;
+                for (int k = 0; k < K; ++k)
+                    acc += (Acc<T>)(Xp[lane_value(kxl, k)] * T(sx)) * (Acc<T>)(Yp[lane_value(kyl, k)] * T(sy));
+                const T v = (T)acc;
+                Cb[clane | cj] = v;
+                if (SCALED) m = fmaxf(m, (float)absval(v));
+            }
+        }
+        if (SCALED) {
+            m = wave_max(m);
+            if (lane == 0) {
+                note_exp(d->cs, b, e);
+                note_max(d->cs, b, m);
+            }
+        }
     }
 }
 
@@ -1322,9 +1429,9 @@ __device__ __forceinline__ void permute_tiles_vec(const to_perm_desc* __restrict
 }
 
 template <typename T, int NU>
-__global__ __launch_bounds__(kBlock) void k_permute_vec_desc(const to_perm_desc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kBlock) void k_permute_vec_desc(const to_perm_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const to_perm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const to_perm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     permute_tiles_vec<T, NU>(d, blockIdx.x - d->block0, d->nblocks, smem);
 }
 
@@ -1337,9 +1444,9 @@ __global__ __launch_bounds__(kBlock) void k_permute_vec(const to_perm_desc desc)
 // descriptor-driven bit permutation, used to bring the two operands of a dense join into
 // row-major [free][shared] form for the MFMA product
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_permute_desc(const to_perm_desc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kBlock) void k_permute_desc(const to_perm_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const to_perm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const to_perm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     permute_tiles<T>(d, blockIdx.x - d->block0, d->nblocks, smem);
 }
 
@@ -1440,12 +1547,12 @@ __device__ __forceinline__ void flush_wide(typename Mfma<T>::V (&acc)[N][M], dou
 }
 
 template <typename T, bool SCALED, bool WIDE>
-__global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
     constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
     __shared__ T As[BK][BM + PAD];
     __shared__ T Bs[BK][BN + PAD];
     typedef typename Mfma<T>::V V;
-    const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int64_t K = 1ll << d->klog;
@@ -1538,7 +1645,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
 // being read, so one barrier per K step suffices (the form for one block per CU, where no
 // second block covers a barrier wait).
 template <typename T, bool SCALED, bool WIDE, int BN, bool TR, bool DB>
-__device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ descs, int ndesc) {
+__device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
     constexpr int BM = 128;
     constexpr int NBUF = DB ? 2 : 1;
     constexpr int NJ = BN / 32;                      // 16-column MFMA tiles per wave
@@ -1553,7 +1660,7 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
     __shared__ __align__(16) T Bs[NBUF][BN][LDK];
     typedef typename Mfma<T>::V V;
     typedef T L __attribute__((ext_vector_type(VE)));
-    const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int64_t K = 1ll << d->klog;
@@ -1699,16 +1806,16 @@ __device__ __forceinline__ void gemm128_body(const to_gemm_desc* __restrict__ de
 // 128x128 tiles with float accumulators, and 128x64 tiles for the wide (double) form,
 // both at two waves per SIMD
 template <typename T, bool SCALED, bool WIDE, bool TR>
-__global__ __launch_bounds__(kBlock, 2) void k_gemm128(const to_gemm_desc* __restrict__ descs, int ndesc) {
-    gemm128_body<T, SCALED, WIDE, WIDE ? 64 : 128, TR, false>(descs, ndesc);
+__global__ __launch_bounds__(kBlock, 2) void k_gemm128(const to_gemm_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
+    gemm128_body<T, SCALED, WIDE, WIDE ? 64 : 128, TR, false>(descs, ndesc, bmap);
 }
 
 // 128x128 tiles at one block (one wave per SIMD) per CU: the register file of a SIMD
 // (accumulation registers included) holds the float and double accumulators of a 64x64
 // quarter, so the wide form keeps the 128-column tile; LDS double-buffered
 template <typename T, bool SCALED, bool WIDE, bool TR>
-__global__ __launch_bounds__(kBlock, 1) void k_gemm128_db(const to_gemm_desc* __restrict__ descs, int ndesc) {
-    gemm128_body<T, SCALED, WIDE, 128, TR, true>(descs, ndesc);
+__global__ __launch_bounds__(kBlock, 1) void k_gemm128_db(const to_gemm_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
+    gemm128_body<T, SCALED, WIDE, 128, TR, true>(descs, ndesc, bmap);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1786,7 +1893,7 @@ __device__ __forceinline__ void x6_interleave() {
 }
 
 template <bool SCALED, bool WIDE, bool TR, int BM>
-__device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ descs, int ndesc) {
+__device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
     constexpr int BN = 128, BK = 32;
     constexpr int NW = x6_waves<WIDE>();             // waves per block
     static_assert(NW == 8 && (BM == 128 || (BM == 256 && !WIDE)), "tile shapes");
@@ -1806,7 +1913,7 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
     typedef float L __attribute__((ext_vector_type(4)));
     typedef uint32_t U2 __attribute__((ext_vector_type(2)));
     __shared__ __align__(16) unsigned char smem[2 * BUF];  // [buf][A planes][B planes]
-    const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x);
+    const to_gemm_desc* d = descs + find_desc(descs, ndesc, blockIdx.x, bmap);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave / WN, wn = wave % WN;
     const int64_t K = 1ll << d->klog;
@@ -2017,8 +2124,8 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
 
 // one block per CU, LDS double-buffered
 template <bool SCALED, bool WIDE, bool TR, int BM>
-__global__ __launch_bounds__(64 * x6_waves<WIDE>(), 1) void k_gemm_x6(const to_gemm_desc* __restrict__ descs, int ndesc) {
-    gemm_x6_body<SCALED, WIDE, TR, BM>(descs, ndesc);
+__global__ __launch_bounds__(64 * x6_waves<WIDE>(), 1) void k_gemm_x6(const to_gemm_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
+    gemm_x6_body<SCALED, WIDE, TR, BM>(descs, ndesc, bmap);
 }
 
 // ---------------------------------------------------------------------------------
@@ -2032,11 +2139,13 @@ int grid_ok(const to_launch& L) {
 template <typename T, bool SCALED>
 int launch_one(const to_launch& L, hipStream_t st) {
     const dim3 grid(L.grid), block(kBlock);
+    // descriptor-table launches: optional block -> descriptor map (int32 per block)
+    const int32_t* bmap = reinterpret_cast<const int32_t*>(L.aux1);
     switch (L.op) {
         case TO_OP_LEAF_GATHER:
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
             hipLaunchKernelGGL((k_leaf_gather<T, SCALED>), grid, block, 0, st,
-                               reinterpret_cast<const to_leaf_desc*>(L.descs), L.count,
+                               reinterpret_cast<const to_leaf_desc*>(L.descs), L.count, bmap,
                                reinterpret_cast<const int64_t*>(L.aux3));
             break;
         case TO_OP_JOIN_STREAM: {
@@ -2045,10 +2154,10 @@ int launch_one(const to_launch& L, hipStream_t st) {
             switch (L.variant) {
 #define TO_STREAM_CASE(YC)                                                                         \
     case YC:                                                                                       \
-        hipLaunchKernelGGL((k_join_stream<T, SCALED, YC>), grid, block, L.lds, st, d, L.count);     \
+        hipLaunchKernelGGL((k_join_stream<T, SCALED, YC>), grid, block, L.lds, st, d, L.count, bmap);     \
         break;                                                                                     \
     case TO_STREAM_VEC | YC:                                                                       \
-        hipLaunchKernelGGL((k_join_stream_vec<T, SCALED, YC>), grid, block, L.lds, st, d, L.count); \
+        hipLaunchKernelGGL((k_join_stream_vec<T, SCALED, YC>), grid, block, L.lds, st, d, L.count, bmap); \
         break;
                 TO_STREAM_CASE(1)
                 TO_STREAM_CASE(2)
@@ -2057,10 +2166,17 @@ int launch_one(const to_launch& L, hipStream_t st) {
                 TO_STREAM_CASE(16)
 #undef TO_STREAM_CASE
                 case TO_STREAM_ROWS:
-                    hipLaunchKernelGGL((k_join_rows<T, SCALED>), grid, block, 0, st, d, L.count);
+                    hipLaunchKernelGGL((k_join_rows<T, SCALED>), grid, block, 0, st, d, L.count, bmap);
                     break;
                 case TO_STREAM_ANY:
-                    hipLaunchKernelGGL((k_join_stream_any<T, SCALED>), grid, block, L.lds, st, d, L.count);
+                    hipLaunchKernelGGL((k_join_stream_any<T, SCALED>), grid, block, L.lds, st, d, L.count, bmap);
+                    break;
+                case TO_STREAM_WAVE:
+                    if ((int64_t)L.grid * (kBlock / 64) < L.count) {
+                        set_err("stream join: a wave launch needs one wave per descriptor");
+                        return TO_ERR_INVALID;
+                    }
+                    hipLaunchKernelGGL((k_join_wave<T, SCALED>), grid, block, 0, st, d, L.count);
                     break;
                 default:
                     set_err("stream join: unsupported y chunk");
@@ -2071,16 +2187,16 @@ int launch_one(const to_launch& L, hipStream_t st) {
         case TO_OP_JOIN_SPLITK:
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
             hipLaunchKernelGGL((k_join_splitk<T, SCALED>), grid, block, 0, st,
-                               reinterpret_cast<const to_join_desc*>(L.descs), L.count);
+                               reinterpret_cast<const to_join_desc*>(L.descs), L.count, bmap);
             break;
         case TO_OP_JOIN_FINISH:
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
             if (L.variant == 1)
                 hipLaunchKernelGGL((k_join_finish_wide<T, SCALED>), grid, block, 0, st,
-                                   reinterpret_cast<const to_join_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_join_desc*>(L.descs), L.count, bmap);
             else
                 hipLaunchKernelGGL((k_join_finish<T, SCALED>), grid, block, 0, st,
-                                   reinterpret_cast<const to_join_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_join_desc*>(L.descs), L.count, bmap);
             break;
         case TO_OP_SCALE_CLEAR:
             if (!L.aux0 || L.iaux0 <= 0) return TO_ERR_INVALID;
@@ -2105,47 +2221,47 @@ int launch_one(const to_launch& L, hipStream_t st) {
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
             if (L.variant == 1)
                 hipLaunchKernelGGL((k_permute_vec_desc<T, kPermVecBits<T>>), grid, block, L.lds, st,
-                                   reinterpret_cast<const to_perm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_perm_desc*>(L.descs), L.count, bmap);
             else if (L.variant == 3)
                 hipLaunchKernelGGL((k_permute_vec_desc<T, kPermVecBits<T> + 1>), grid, block, L.lds, st,
-                                   reinterpret_cast<const to_perm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_perm_desc*>(L.descs), L.count, bmap);
             else
                 hipLaunchKernelGGL((k_permute_desc<T>), grid, block, L.lds, st,
-                                   reinterpret_cast<const to_perm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_perm_desc*>(L.descs), L.count, bmap);
             break;
         case TO_OP_GEMM:
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
             if (L.variant == 1)
                 hipLaunchKernelGGL((k_gemm128<T, SCALED, false, false>), grid, block, 0, st,
-                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count, bmap);
             else if (L.variant == 3)
                 hipLaunchKernelGGL((k_gemm128<T, SCALED, true, false>), grid, block, 0, st,
-                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count, bmap);
             else if (L.variant == 5)
                 hipLaunchKernelGGL((k_gemm128<T, SCALED, false, true>), grid, block, 0, st,
-                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count, bmap);
             else if (L.variant == 7)
                 hipLaunchKernelGGL((k_gemm128<T, SCALED, true, true>), grid, block, 0, st,
-                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count, bmap);
             else if (L.variant == 9)
                 hipLaunchKernelGGL((k_gemm128_db<T, SCALED, false, false>), grid, block, 0, st,
-                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count, bmap);
             else if (L.variant == 11)
                 hipLaunchKernelGGL((k_gemm128_db<T, SCALED, true, false>), grid, block, 0, st,
-                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count, bmap);
             else if (L.variant == 13)
                 hipLaunchKernelGGL((k_gemm128_db<T, SCALED, false, true>), grid, block, 0, st,
-                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count, bmap);
             else if (L.variant == 15)
                 hipLaunchKernelGGL((k_gemm128_db<T, SCALED, true, true>), grid, block, 0, st,
-                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count, bmap);
             else if (L.variant & 16) {
                 if constexpr (sizeof(T) == 4) {
                     const to_gemm_desc* gd = reinterpret_cast<const to_gemm_desc*>(L.descs);
                     switch (L.variant) {
 #define TO_X6_CASE(V, W, TRN, BMT)                                                                              \
     case V:                                                                                                     \
-        hipLaunchKernelGGL((k_gemm_x6<SCALED, W, TRN, BMT>), grid, dim3(64 * x6_waves<W>()), 0, st, gd, L.count); \
+        hipLaunchKernelGGL((k_gemm_x6<SCALED, W, TRN, BMT>), grid, dim3(64 * x6_waves<W>()), 0, st, gd, L.count, bmap); \
         break;
                         TO_X6_CASE(25, false, false, 128)
                         TO_X6_CASE(27, true, false, 128)
@@ -2165,10 +2281,10 @@ int launch_one(const to_launch& L, hipStream_t st) {
             }
             else if (L.variant == 0)
                 hipLaunchKernelGGL((k_gemm<T, SCALED, false>), grid, block, 0, st,
-                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count, bmap);
             else if (L.variant == 2)
                 hipLaunchKernelGGL((k_gemm<T, SCALED, true>), grid, block, 0, st,
-                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count);
+                                   reinterpret_cast<const to_gemm_desc*>(L.descs), L.count, bmap);
             else {
                 set_err("gemm: unknown tile variant");
                 return TO_ERR_INVALID;
@@ -2182,7 +2298,7 @@ int launch_one(const to_launch& L, hipStream_t st) {
         case TO_OP_LEAF_BUILD:
             if (!L.descs || L.count <= 0) return TO_ERR_INVALID;
             hipLaunchKernelGGL((k_leaf_build<T>), grid, block, 0, st,
-                               reinterpret_cast<const to_leaf_build_desc*>(L.descs), L.count);
+                               reinterpret_cast<const to_leaf_build_desc*>(L.descs), L.count, bmap);
             break;
         case TO_OP_BATCH_BEGIN:
             if (!L.aux2 || L.iaux0 <= 0) return TO_ERR_INVALID;

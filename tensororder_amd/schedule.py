@@ -31,6 +31,10 @@ STREAM_MAX_S = 12
 KIND_ORDER = {"permute": 0, "stream": 1, "splitk": 1, "gemm": 1, "finish": 2}
 LDS_LIMIT = 64 * 1024
 MAX_STREAM_BLOCKS = 4096
+# one-wave streamed joins (many-instance batch): at most this many products per lane
+WAVE_MAX_LANE_PRODUCTS = 64
+# the many-instance batch takes the row form for joins of at least this many output bits
+ROWS_MIN_BITS_BATCH = 20
 
 
 class _Arena:
@@ -261,7 +265,10 @@ def perm_key(esize, rank, nb):
 
 
 def stream_variant(v, g):
-    """Launch variant of a streamed join: YC, TO_STREAM_VEC | YC, or TO_STREAM_ROWS."""
+    """Launch variant of a streamed join: YC, TO_STREAM_VEC | YC, TO_STREAM_ROWS or
+    TO_STREAM_WAVE."""
+    if v.get("wave"):
+        return _lib.TO_STREAM_WAVE
     if v.get("rows"):
         return _lib.TO_STREAM_ROWS
     return v["yc"] | (_lib.TO_STREAM_VEC if v["vec"] else 0)
@@ -322,12 +329,17 @@ def choose_variant(g, nb, esize, launch_bound=False):
                     | (16 if x6 else 0) | (32 if tall else 0),
                     nblocks=int(min(tiles, cap)),
                     perm_x=not _canonical(g["xf"], g["xs"]), perm_y=not _canonical(g["yf"], g["ys"]))
+    # the many-instance batch: a join whose outputs need at most WAVE_MAX_LANE_PRODUCTS
+    # products per lane of one wave runs as one wave, four joins to a block (the kernel
+    # holds the 2^s shared offsets one per lane: s <= 6)
+    if launch_bound and s <= 6 and (-(-(nb << (fx + fy)) // 64) << s) <= WAVE_MAX_LANE_PRODUCTS:
+        return dict(kind="stream", wave=True, yc=1, nych=1, lds=0, stage=0, vec=False, nblocks=1)
     yc = min(16, 1 << fy)
     stage = 0
     # row form: outputs C[f][y] with 2^fy >= 32 contiguous y and few shared bits, written
     # as whole rows (each thread keeps 16 bytes of y and their Y values in registers)
     if STREAM_ROWS and g["shy"] == 0 and g["shf"] == fy and 5 <= fy <= (10 if esize == 4 else 9) \
-            and fx >= 6 and s <= 4 and not (launch_bound and fx + fy + math.log2(max(nb, 1)) < 20):
+            and fx >= 6 and s <= 4 and not (launch_bound and fx + fy + math.log2(max(nb, 1)) < ROWS_MIN_BITS_BATCH):
         lr = min(fx, 6, 11 + (2 if esize == 4 else 1) - fy)  # log2 rows per tile (<= 8 passes)
         return dict(kind="stream", yc=16, nych=1, lds=0, stage=0, vec=False, rows=True,
                     nblocks=int(min(nb << (fx - lr), MAX_STREAM_BLOCKS)))
@@ -517,13 +529,14 @@ def perm_plan(rank, out_pos, max_tile_bits=12):
 PERM_VEC_BITS = {4: 13, 8: 12}
 
 
-def perm_desc(src, dst, rank, out_pos, nbatch, stride, esize):
+def perm_desc(src, dst, rank, out_pos, nbatch, stride, esize, big_tiles=True):
     """(to_perm_desc, grid, LDS bytes, variant) of a batched bit permutation; variants 1
     and 3 are the vectorised kernel (tensors of at least its tile bits, 16-byte aligned
-    buffers) with 32 KB and 64 KB tiles."""
+    buffers) with 32 KB and 64 KB tiles.  ``big_tiles=False`` keeps to 32 KB tiles, so that
+    the permutes of a many-instance step share one launch."""
     vec_bits = PERM_VEC_BITS[esize]
     vec = rank >= vec_bits and src % 16 == 0 and dst % 16 == 0 and (stride * esize) % 16 == 0
-    big = vec and rank > vec_bits
+    big = vec and big_tiles and rank > vec_bits
     u_in, v_out, v2u, rest_in, rest_out = perm_plan(rank, out_pos, (vec_bits + big) if vec else 12)
     d = _lib.PermDesc()
     d.in_, d.out = src, dst
@@ -729,10 +742,11 @@ def peak_bytes(nodes, tables, W, entry_type):
     return choose_order(nodes, tables, W, esize)[1]
 
 
-def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
+def join_records(n, nodes, W, esize, scaled, abase, scratch, recp, launch_bound=False):
     """Descriptor records of one join, in execution order:
     [(kind, op, variant, lds, descriptor struct, blocks)].  Descriptors carry block0 = 0;
-    a grouped launch rebases them."""
+    a grouped launch rebases them.  ``launch_bound`` (the many-instance batch) keeps every
+    operand permute on one tile size."""
     g, v = n.geom, n.geom["var"]
     nb = W if n.batched else 1
     X, Y = (nodes[n.left], nodes[n.right]) if g["x_is_left"] else (nodes[n.right], nodes[n.left])
@@ -749,7 +763,8 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp):
                 out_pos[p] = t
             for i, p in enumerate(fpos):
                 out_pos[p] = len(spos) + i
-            pd, grid, lds, pvar = perm_desc(op.ptr, dst, op.rank, out_pos, nb_op, 1 << op.rank, esize)
+            pd, grid, lds, pvar = perm_desc(op.ptr, dst, op.rank, out_pos, nb_op, 1 << op.rank, esize,
+                                               big_tiles=not launch_bound)
             out.append(("permute", _lib.TO_OP_PERMUTE, pvar, lds, pd, grid))
             operands.append((dst, stride, op, True))
         else:

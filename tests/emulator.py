@@ -82,6 +82,8 @@ class Emulator:
         self.heap = heap
         self.scaled = entry_type == _lib.TO_F32S
         self.dt = np.float32 if self.scaled else np.float64
+        self.maps_checked = 0
+        self.wave_joins = 0
 
     def recs(self, addr, count):
         raw = self.heap.view(addr, np.uint8, 16 * count).reshape(count, 16)
@@ -100,8 +102,25 @@ class Emulator:
             new = max(cur, int(maxbits))
             self.heap.mem[off + 8:off + 12] = np.frombuffer(np.uint32(new).tobytes(), np.uint8)
 
+    TABLE_DESCS = {1: "LeafDesc", 2: "JoinDesc", 3: "JoinDesc", 4: "JoinDesc", 9: "PermDesc", 10: "GemmDesc",
+                   13: "LeafBuildDesc"}
+
+    def check_block_map(self, L):
+        """A launch's block -> descriptor map (aux1) must name, for every block, the last
+        descriptor whose block0 is at or below it -- what the kernels' binary search finds."""
+        cls = getattr(_lib, self.TABLE_DESCS[L.op])
+        size = ctypes.sizeof(cls)
+        b0 = np.array([self.heap.read_struct(cls, L.descs + i * size).block0 for i in range(L.count)])
+        assert b0[0] == 0 and np.all(np.diff(b0) >= 0)
+        bmap = self.heap.view(L.aux1, np.int32, L.grid)
+        want = np.searchsorted(b0, np.arange(L.grid), side="right") - 1
+        assert np.array_equal(bmap, want), "block map disagrees with block0"
+        self.maps_checked += 1
+
     def run(self, launches):
         for L in launches:
+            if L.op in self.TABLE_DESCS and L.aux1:
+                self.check_block_map(L)
             getattr(self, "op_%d" % L.op)(L)
 
     # leaf gather
@@ -155,10 +174,17 @@ class Emulator:
 
     def op_2(self, L):
         valid = {y | f for y in (1, 2, 4, 8, 16) for f in (0, _lib.TO_STREAM_VEC)} | {_lib.TO_STREAM_ROWS}
+        if L.variant == _lib.TO_STREAM_WAVE:
+            # one wave per descriptor, four per block: the grid covers every descriptor
+            assert L.grid * 4 >= L.count and L.lds == 0 and not L.aux1
+            self.wave_joins += L.count
         for i in range(L.count):
             d = self.heap.read_struct(_lib.JoinDesc, L.descs + i * ctypes.sizeof(_lib.JoinDesc))
-            # a launch of one form carries it in every descriptor; a mixed launch names each
-            assert d.kvar in valid and (L.variant == _lib.TO_STREAM_ANY or d.kvar == L.variant)
+            if L.variant == _lib.TO_STREAM_WAVE:
+                assert d.kvar == _lib.TO_STREAM_WAVE
+            else:
+                # a launch of one form carries it in every descriptor; a mixed launch names each
+                assert d.kvar in valid and (L.variant == _lib.TO_STREAM_ANY or d.kvar == L.variant)
             self._join(d)
 
     def op_3(self, L):  # split-K partials: the emulator computes the join at FINISH
@@ -370,4 +396,6 @@ def emulate_batch(instances, entry_type, budget=None):
             acc = heap.read_struct(_lib.Accum, prog.accum.data_ptr() + i * ctypes.sizeof(_lib.Accum))
             fm, fe = prog.leaf_factors[i]
             out.append(math.ldexp(acc.mant * fm, acc.exp + fe))
+    progs[0].maps_checked = emu.maps_checked
+    progs[0].wave_joins = emu.wave_joins
     return out, (progs[0] if budget is None else progs)

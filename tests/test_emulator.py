@@ -145,8 +145,8 @@ def test_memory_plan_never_overlaps_live_buffers(order):
 
 
 @pytest.mark.parametrize("et,tol", [(_lib.TO_F64, 1e-12), (_lib.TO_F32S, 1e-6)])
-def test_batch_of_uneven_instances_emulated(et, tol):
-    from tensororder_amd import network, planners
+def test_batch_of_uneven_instances_emulated(et, tol, monkeypatch):
+    from tensororder_amd import batch, network, planners
     from tensororder_amd.formula import parse_dimacs_text
     from tensororder_amd.instances import vertex_cover_cnf
 
@@ -157,7 +157,20 @@ def test_batch_of_uneven_instances_emulated(et, tol):
         wants.append(numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan)))
         tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
         instances.append((tree, tables))
+    # with one-wave joins turned off, block -> descriptor maps on every grouped launch of
+    # two or more descriptors, each checked by the emulator against the binary search the
+    # kernels do without one
+    with monkeypatch.context() as mp:
+        mp.setattr(batch, "MAP_MIN_DESCS", 2)
+        mp.setattr(schedule, "WAVE_MAX_LANE_PRODUCTS", 0)
+        got, prog = emulator.emulate_batch(instances, et)
+    mapped = sum(1 for L in prog.launches if L.aux1 and L.op in emulator.Emulator.TABLE_DESCS)
+    assert mapped > 0 and prog.maps_checked == mapped and prog.wave_joins == 0
+    for g, w in zip(got, wants):
+        assert g == pytest.approx(w, rel=tol)
+    # as compiled: the smallest streamed joins run one wave each
     got, prog = emulator.emulate_batch(instances, et)
+    assert prog.wave_joins > 0
     assert prog.stats["steps"] < prog.stats["joins"]
     for g, w in zip(got, wants):
         assert g == pytest.approx(w, rel=tol)

@@ -254,6 +254,92 @@ def _gemm_variants(esize, mlog, nlog, klog):
 
 
 @pytest.mark.parametrize("et", ["f64", "f32s"])
+def test_one_wave_stream_joins(hip, hip32, et):
+    # TO_STREAM_WAVE: small joins of several shapes in one launch (one wave each, the last
+    # block partly filled), batched and not, scattered shared bits and both output layouts,
+    # against float64 sums; FP32-scaled outputs carry their exponent and max in the record
+    import ctypes
+
+    import torch
+
+    from tensororder_amd import _lib
+
+    api = hip if et == "f64" else hip32
+    esize = api.get_entry_size()
+    dt = np.float64 if esize == 8 else np.float32
+    rng = np.random.default_rng(5)
+    dev = torch.device("cuda", 0)
+    # (fx, fy, s, nbatch, y bits lowest in the output)
+    geoms = [(2, 1, 1, 1, True), (3, 2, 0, 1, False), (0, 1, 3, 3, True), (5, 1, 2, 2, False),
+             (4, 0, 2, 1, True), (9, 1, 1, 1, True), (6, 0, 0, 1, True)]
+
+    def pdep(v, pos):
+        return sum(((v >> i) & 1) << p for i, p in enumerate(pos))
+
+    descs, keep, cases = [], [], []
+    rsz = ctypes.sizeof(_lib.ScaleRec)
+    for fx, fy, s, nb, yinner in geoms:
+        rx, ry = fx + s, fy + s
+        px, py = [int(v) for v in rng.permutation(rx)], [int(v) for v in rng.permutation(ry)]
+        xf, xs = sorted(px[:fx]), px[fx:]
+        yf, ys = sorted(py[:fy]), py[fy:]
+        X = rng.uniform(-1.0, 2.0, (nb, 1 << rx)).astype(dt)
+        Y = rng.uniform(-1.0, 2.0, (nb, 1 << ry)).astype(dt)
+        tx, ty = torch.from_numpy(X).to(dev), torch.from_numpy(Y).to(dev)
+        tc = torch.full((nb, 1 << (fx + fy)), float("nan"), dtype=tx.dtype, device=dev)
+        d = _lib.JoinDesc()
+        d.x, d.y, d.c = tx.data_ptr(), ty.data_ptr(), tc.data_ptr()
+        d.xb, d.yb, d.cb = 1 << rx, 1 << ry, 1 << (fx + fy)
+        d.fx, d.fy, d.s, d.nbatch = fx, fy, s, nb
+        d.shf, d.shy = (fy, 0) if yinner else (0, fx)
+        d.kvar = _lib.TO_STREAM_WAVE
+        for arr, vals in ((d.xf_pos, xf), (d.xs_pos, xs), (d.yf_pos, yf), (d.ys_pos, ys)):
+            for i, p in enumerate(vals):
+                arr[i] = p
+        recs = None
+        if esize == 4:
+            rs = np.zeros((3 * nb, 2), dtype=np.int64)
+            for b in range(nb):
+                rs[b] = (3, int(np.float32(np.abs(X[b]).max()).view(np.uint32)))
+                rs[nb + b] = (-2, int(np.float32(np.abs(Y[b]).max()).view(np.uint32)))
+            recs = torch.from_numpy(rs).to(dev)
+            d.xs, d.ys, d.cs = recs.data_ptr(), recs.data_ptr() + nb * rsz, recs.data_ptr() + 2 * nb * rsz
+            d.xsb = d.ysb = 1
+        want = np.zeros((nb, 1 << (fx + fy)))
+        for b in range(nb):
+            for f in range(1 << fx):
+                for y in range(1 << fy):
+                    want[b, (f << d.shf) | (y << d.shy)] = sum(
+                        float(X[b, pdep(f, xf) + pdep(k, xs)]) * float(Y[b, pdep(y, yf) + pdep(k, ys)])
+                        for k in range(1 << s))
+        descs.append(d)
+        keep += [tx, ty, tc, recs]
+        cases.append((tc, recs, want, nb))
+    blob = torch.frombuffer(bytearray(b"".join(bytes(d) for d in descs)), dtype=torch.uint8).to(dev)
+    torch.cuda.synchronize()
+    _say("case one-wave joins", et, len(descs))
+    L = _lib.Launch(op=_lib.TO_OP_JOIN_STREAM, variant=_lib.TO_STREAM_WAVE, count=len(descs),
+                    grid=-(-len(descs) // 4), descs=blob.data_ptr())
+    arr = _lib.launch_array([L])
+    _lib.check(api._lib.to_run(ctypes.addressof(arr), 1, api._entry_type,
+                               ctypes.c_void_p(api._stream.cuda_stream)), "to_run")
+    api._stream.synchronize()
+    for (tc, recs, want, nb), geom in zip(cases, geoms):
+        c = tc.cpu().numpy().astype(np.float64)
+        if esize == 8:
+            np.testing.assert_allclose(c, want, rtol=1e-12, atol=1e-300, err_msg=str(geom))
+            continue
+        rs = recs.cpu().numpy()
+        for b in range(nb):
+            # represented value: stored * 2^exp of the output record; operands carry 2^3, 2^-2
+            cexp = int(rs[2 * nb + b, 0])
+            np.testing.assert_allclose(np.ldexp(c[b], cexp), np.ldexp(want[b], 1), rtol=2e-6, atol=1e-30,
+                                       err_msg=str(geom))
+            got_max = int(rs[2 * nb + b, 1]) & 0xFFFFFFFF
+            assert got_max == int(np.float32(np.abs(c[b]).max()).view(np.uint32)), geom
+
+
+@pytest.mark.parametrize("et", ["f64", "f32s"])
 def test_gemm_every_tile_variant(hip, hip32, et):
     # every MFMA product kernel (64x64, 128x128, 128x64 wide, 128x128 one-block-per-CU
     # double-buffered, bf16 three-way split; plain and transposed) on the same operands,

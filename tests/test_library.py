@@ -47,6 +47,11 @@ def test_invalid_records_are_rejected_without_launching():
     arr = _lib.launch_array([neg])
     assert lib.to_run(ctypes.addressof(arr), 1, _lib.TO_F64, None) == _lib.TO_ERR_INVALID
     assert lib.to_run(ctypes.addressof(arr), 1, 7, None) == _lib.TO_ERR_INVALID
+    # a one-wave join launch whose grid holds fewer waves than descriptors
+    waves = _lib.Launch(op=_lib.TO_OP_JOIN_STREAM, variant=_lib.TO_STREAM_WAVE, count=9, grid=2, descs=256)
+    arr = _lib.launch_array([waves])
+    assert lib.to_run(ctypes.addressof(arr), 1, _lib.TO_F64, None) == _lib.TO_ERR_INVALID
+    assert b"wave" in lib.to_last_error()
     perm = (ctypes.c_int32 * 2)(0, 0)
     assert lib.to_permute(ctypes.c_void_p(8), ctypes.c_void_p(8), 2, ctypes.addressof(perm), 8, 1,
                           None) == _lib.TO_ERR_INVALID
