@@ -56,9 +56,44 @@ __device__ __forceinline__ TO_GLOBAL T* gptr(uint64_t p) {
 }
 
 __device__ __forceinline__ int64_t pdep(uint64_t v, const int8_t* pos, int n) {
+    // The position bytes of 8 bits at a time are loaded before any is used (indices
+    // clamped to n - 1, so every load is in bounds and none waits behind a loop exit):
+    // one memory wait per 8 bits rather than one per bit.
     int64_t r = 0;
-    for (int i = 0; i < n; ++i) r |= (int64_t)((v >> i) & 1ull) << pos[i];
+    for (int g = 0; g < n; g += 8) {
+        int p[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) p[j] = pos[g + j < n ? g + j : n - 1];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (g + j < n) r |= (int64_t)((v >> (g + j)) & 1ull) << p[j];
+    }
     return r;
+}
+
+// Bit positions spread across the lanes of a wave: lane i < n holds 1 << pos[i], loaded
+// with one memory access per lane instead of one dependent access per bit.  Made where
+// every lane of the wave runs (v_readlane reads a lane whether or not it is active
+// later); a deposit then costs a few ALU instructions per bit and no memory wait.
+struct LaneBits {
+    uint32_t lo, hi;
+};
+
+__device__ __forceinline__ LaneBits lane_bits(const int8_t* pos, int n) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t b = lane < n ? 1ull << pos[lane] : 0ull;
+    return LaneBits{(uint32_t)b, (uint32_t)(b >> 32)};
+}
+
+// pdep(v, pos + first, n) from lane-held positions (first + n <= 64)
+__device__ __forceinline__ int64_t pdep_l(uint64_t v, LaneBits p, int first, int n) {
+    uint32_t lo = 0, hi = 0;
+    for (int i = 0; i < n; ++i) {
+        const uint32_t m = 0u - (uint32_t)((v >> i) & 1ull);
+        lo |= m & (uint32_t)__builtin_amdgcn_readlane((int)p.lo, first + i);
+        hi |= m & (uint32_t)__builtin_amdgcn_readlane((int)p.hi, first + i);
+    }
+    return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 // a value known to be the same in every lane, moved to scalar registers
@@ -222,7 +257,8 @@ __global__ __launch_bounds__(kBlock) void k_leaf_build(const to_leaf_build_desc*
 // ---------------------------------------------------------------------------------
 // streaming join: each thread owns one free index f of X and YC free indices y of Y.
 // LDS: kx[2^s], ky[2^s] (uint32 offsets of the shared bits), then (fx >= 8 only) the
-// Y slab ylds[YC][2^s] of the current (batch item, y chunk).
+// Y slab ylds[YC][2^s] of the current (batch item, y chunk), held as doubles so the inner
+// loop converts only the X values.
 
 template <typename T, bool SCALED, int YC>
 __device__ __forceinline__ void join_stream_body(const to_join_desc* __restrict__ d, const int local,
@@ -242,10 +278,12 @@ __device__ __forceinline__ void join_stream_body(const to_join_desc* __restrict_
 
     uint32_t* kx = reinterpret_cast<uint32_t*>(smem);
     uint32_t* ky = kx + K;
-    T* ylds = reinterpret_cast<T*>(ky + K);
+    Acc<T>* ylds = reinterpret_cast<Acc<T>*>(ky + K);  // held as the accumulator type
+    const LaneBits XF = lane_bits(d->xf_pos, fx), XS = lane_bits(d->xs_pos, s);
+    const LaneBits YF = lane_bits(d->yf_pos, fy), YS = lane_bits(d->ys_pos, s);
     for (int k = tid; k < K; k += kBlock) {
-        kx[k] = (uint32_t)pdep(k, d->xs_pos, s);
-        ky[k] = (uint32_t)pdep(k, d->ys_pos, s);
+        kx[k] = (uint32_t)pdep_l(k, XS, 0, s);
+        ky[k] = (uint32_t)pdep_l(k, YS, 0, s);
     }
     __syncthreads();
 
@@ -260,20 +298,22 @@ __device__ __forceinline__ void join_stream_body(const to_join_desc* __restrict_
         const int64_t nft = 1ll << hb;
         const int64_t ntiles = (int64_t)nbatch << (hb + lnych);
         const int64_t t0 = ntiles * local / nblk, t1 = ntiles * (local + 1) / nblk;
-        const int64_t xlo = pdep(tid, d->xf_pos, 8);
+        const int64_t xlo = pdep_l(tid, XF, 0, 8);
         // free-bit positions increase with the bit index, so the offsets of consecutive ft
         // follow x' = ((x | ~mask) + 1) & mask
-        const int64_t hmask = pdep(~0ull, d->xf_pos + 8, hb);
+        const int64_t hmask = pdep_l(~0ull, XF, 8, hb);
         int ych = (int)(t0 & (nych - 1));
         int64_t ft = (t0 >> lnych) & (nft - 1);
         int64_t batch = t0 >> (lnych + hb);
-        int64_t xhi = pdep(ft, d->xf_pos + 8, hb);
+        int64_t xhi = pdep_l(ft, XF, 8, hb);
         int64_t staged = -1;
         // Y-free bits below the f bits, all of them in one chunk: stage the outputs
         const bool stage_out = (d->stage & 1) != 0;
         const bool stage_in = (d->stage & 2) != 0;
-        T* otile = ylds + YC * K;
+        T* otile = reinterpret_cast<T*>(ylds + YC * K);
         T* xtile = otile + (stage_out ? (kBlock << fy) : 0);
+        // y bits lowest in C, C and its batch stride 16-byte aligned
+        const bool vec_c = shy == 0 && ((d->c | (uint64_t)(cb * (int64_t)sizeof(T))) & 15) == 0;
         float mrun = 0.0f;
         for (int64_t t = t0; t < t1; ++t) {
             const int64_t key = batch * nych + ych;
@@ -291,8 +331,8 @@ __device__ __forceinline__ void join_stream_body(const to_join_desc* __restrict_
                 for (int i = tid; i < YC * K; i += kBlock) {
                     const int yc = i >> s, k = i & (K - 1);
                     T v = T(0);
-                    if (yc < ny) v = Yb[pdep(ych * YC + yc, d->yf_pos, fy) + ky[k]] * T(sy);
-                    ylds[i] = v;
+                    if (yc < ny) v = Yb[pdep_l(ych * YC + yc, YF, 0, fy) + ky[k]] * T(sy);
+                    ylds[i] = (Acc<T>)v;
                 }
                 __syncthreads();
                 staged = key;
@@ -344,6 +384,19 @@ __device__ __forceinline__ void join_stream_body(const to_join_desc* __restrict_
                 __syncthreads();
                 auto dst = Cb + (ft << (8 + fy));
                 for (int i = tid; i < (kBlock << fy); i += kBlock) dst[i] = otile[i];
+            } else if (YC * sizeof(T) >= 16 && vec_c && ny == YC) {
+                // the chunk's y values are consecutive entries (y bits lowest): 16-byte
+                // stores, aligned since YC * ych and 2^shf >= YC are multiples of the vector
+                typedef T V16 __attribute__((ext_vector_type(16 / sizeof(T))));
+                constexpr int VE = 16 / sizeof(T);
+                auto dst = Cb + (f << shf) + (int64_t)ych * YC;
+#pragma unroll
+                for (int yc = 0; yc < YC; yc += VE) {
+                    V16 w;
+#pragma unroll
+                    for (int e = 0; e < VE; ++e) w[e] = (T)acc[yc + e];
+                    *reinterpret_cast<TO_GLOBAL V16*>(dst + yc) = w;
+                }
             } else {
 #pragma unroll
                 for (int yc = 0; yc < YC; ++yc) {
@@ -383,8 +436,9 @@ __device__ __forceinline__ void join_stream_body(const to_join_desc* __restrict_
         const int64_t nLt = (nL + kBlock - 1) / kBlock;
         const int64_t ntiles = nLt << lnych;
         const int64_t t0 = ntiles * local / nblk, t1 = ntiles * (local + 1) / nblk;
-        const int64_t xoff = pdep(tid & fmask, d->xf_pos, fx);
+        const int64_t xoff = pdep_l(tid & fmask, XF, 0, fx);
         const int f = tid & fmask;
+        const bool vec_c = shy == 0 && ((d->c | (uint64_t)(cb * (int64_t)sizeof(T))) & 15) == 0;
         int ych = (int)(t0 & (nych - 1));
         int64_t lt = t0 >> lnych;
         for (int64_t t = t0; t < t1; ++t) {
@@ -404,7 +458,7 @@ __device__ __forceinline__ void join_stream_body(const to_join_desc* __restrict_
                 const auto Yb = Y + batch * yb;
                 int64_t yoff[YC];
 #pragma unroll
-                for (int yc = 0; yc < YC; ++yc) yoff[yc] = (yc < ny) ? pdep(ych * YC + yc, d->yf_pos, fy) : 0;
+                for (int yc = 0; yc < YC; ++yc) yoff[yc] = (yc < ny) ? pdep_l(ych * YC + yc, YF, 0, fy) : 0;
                 Acc<T> acc[YC];
 #pragma unroll
                 for (int yc = 0; yc < YC; ++yc) acc[yc] = 0.0;
@@ -416,13 +470,30 @@ __device__ __forceinline__ void join_stream_body(const to_join_desc* __restrict_
                     }
                 }
                 auto Cb = C + batch * cb;
+                if (YC * sizeof(T) >= 16 && vec_c && ny == YC) {
+                    // consecutive y entries (y bits lowest): 16-byte stores
+                    typedef T V16 __attribute__((ext_vector_type(16 / sizeof(T))));
+                    constexpr int VE = 16 / sizeof(T);
+                    auto dst = Cb + ((int64_t)f << shf) + (int64_t)ych * YC;
 #pragma unroll
-                for (int yc = 0; yc < YC; ++yc) {
-                    if (yc < ny) {
-                        const int64_t y = (int64_t)ych * YC + yc;
-                        const T v = (T)acc[yc];
-                        Cb[((int64_t)f << shf) | (y << shy)] = v;
-                        if (SCALED) m = fmaxf(m, (float)absval(v));
+                    for (int yc = 0; yc < YC; yc += VE) {
+                        V16 w;
+#pragma unroll
+                        for (int e = 0; e < VE; ++e) {
+                            w[e] = (T)acc[yc + e];
+                            if (SCALED) m = fmaxf(m, (float)absval(w[e]));
+                        }
+                        *reinterpret_cast<TO_GLOBAL V16*>(dst + yc) = w;
+                    }
+                } else {
+#pragma unroll
+                    for (int yc = 0; yc < YC; ++yc) {
+                        if (yc < ny) {
+                            const int64_t y = (int64_t)ych * YC + yc;
+                            const T v = (T)acc[yc];
+                            Cb[((int64_t)f << shf) | (y << shy)] = v;
+                            if (SCALED) m = fmaxf(m, (float)absval(v));
+                        }
                     }
                 }
                 if (SCALED && f == 0 && ych == 0) note_exp(d->cs, batch, ex + ey);
@@ -761,9 +832,11 @@ __device__ __forceinline__ void join_stream_vec_body(const to_join_desc* __restr
     uint32_t* kx = reinterpret_cast<uint32_t*>(smem);
     uint32_t* ky = kx + K;
     T* ylds = reinterpret_cast<T*>(smem + ((8 * K + 15) & ~15));
+    const LaneBits XF = lane_bits(d->xf_pos, fx), XS = lane_bits(d->xs_pos, s);
+    const LaneBits YF = lane_bits(d->yf_pos, fy), YS = lane_bits(d->ys_pos, s);
     for (int k = tid; k < K; k += kBlock) {
-        kx[k] = (uint32_t)pdep(k, d->xs_pos, s);
-        ky[k] = (uint32_t)pdep(k, d->ys_pos, s);
+        kx[k] = (uint32_t)pdep_l(k, XS, 0, s);
+        ky[k] = (uint32_t)pdep_l(k, YS, 0, s);
     }
     __syncthreads();
 
@@ -773,11 +846,11 @@ __device__ __forceinline__ void join_stream_vec_body(const to_join_desc* __restr
     const int64_t nft = 1ll << hb;
     const int64_t ntiles = (int64_t)nbatch << hb;
     const int64_t t0 = ntiles * local / nblk, t1 = ntiles * (local + 1) / nblk;
-    const int64_t xlo = pdep((uint64_t)tid << LVF, d->xf_pos, tb);
-    const int64_t hmask = pdep(~0ull, d->xf_pos + tb, hb);
+    const int64_t xlo = pdep_l((uint64_t)tid << LVF, XF, 0, tb);
+    const int64_t hmask = pdep_l(~0ull, XF, tb, hb);
     int64_t ft = t0 & (nft - 1);
     int64_t batch = t0 >> hb;
-    int64_t xhi = pdep(ft, d->xf_pos + tb, hb);
+    int64_t xhi = pdep_l(ft, XF, tb, hb);
     int64_t staged = -1;
     float sx = 1.0f, sy = 1.0f;
     int64_t ex = 0, ey = 0;
@@ -793,7 +866,7 @@ __device__ __forceinline__ void join_stream_vec_body(const to_join_desc* __restr
             const auto Yb = Y + batch * yb;
             for (int i = tid; i < YC * K; i += kBlock) {
                 const int yc = i >> s, k = i & (K - 1);
-                ylds[i] = Yb[pdep(yc, d->yf_pos, fy) + ky[k]] * T(sy);
+                ylds[i] = Yb[pdep_l(yc, YF, 0, fy) + ky[k]] * T(sy);
             }
             __syncthreads();
             staged = batch;
@@ -930,9 +1003,11 @@ __device__ __forceinline__ void join_rows_body(const to_join_desc* __restrict__ 
     const TO_GLOBAL T* __restrict__ X = gptr<const T>(d->x);
     const TO_GLOBAL T* __restrict__ Y = gptr<const T>(d->y);
     TO_GLOBAL T* __restrict__ C = gptr<T>(d->c);
+    const LaneBits XF = lane_bits(d->xf_pos, fx), XS = lane_bits(d->xs_pos, S);
+    const LaneBits YF = lane_bits(d->yf_pos, fy), YS = lane_bits(d->ys_pos, S);
     int64_t kx[KMAX];
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) kx[k] = k < K ? uniform64(pdep(k, d->xs_pos, S)) : 0;
+    for (int k = 0; k < KMAX; ++k) kx[k] = k < K ? uniform64(pdep_l(k, XS, 0, S)) : 0;
     // a tile is 64 rows, or 8 passes of rb rows when that is fewer: a block's passes are
     // sequential (each waits for its X loads), so wide rows get short tiles, more blocks
     const int lr = min(min(fx, 6), 11 + (sizeof(T) == 4 ? 2 : 1) - fy);  // log2 rows per tile
@@ -941,8 +1016,8 @@ __device__ __forceinline__ void join_rows_body(const to_join_desc* __restrict__ 
     const int64_t nft = 1ll << tb;
     const int64_t ntiles = (int64_t)d->nbatch << tb;
     const int64_t t0 = ntiles * local / d->nblocks, t1 = ntiles * (local + 1) / d->nblocks;
-    const int64_t fmask = pdep(~0ull, d->xf_pos, fx);
-    const int64_t xstep = pdep((uint64_t)rb, d->xf_pos, fx);  // rb rows on, in X's bit layout
+    const int64_t fmask = pdep_l(~0ull, XF, 0, fx);
+    const int64_t xstep = pdep_l((uint64_t)rb, XF, 0, fx);  // rb rows on, in X's bit layout
     T yr[KMAX][VY];
     int64_t staged = -1;
     float sxy = 1.0f;
@@ -965,9 +1040,9 @@ __device__ __forceinline__ void join_rows_body(const to_join_desc* __restrict__ 
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) {
                 if (k < K) {
-                    const int64_t ky = pdep(k, d->ys_pos, S);
+                    const int64_t ky = pdep_l(k, YS, 0, S);
 #pragma unroll
-                    for (int e = 0; e < VY; ++e) yr[k][e] = Yb[pdep(j * VY + e, d->yf_pos, fy) + ky];
+                    for (int e = 0; e < VY; ++e) yr[k][e] = Yb[pdep_l(j * VY + e, YF, 0, fy) + ky];
                 }
             }
             staged = batch;
@@ -976,7 +1051,7 @@ __device__ __forceinline__ void join_rows_body(const to_join_desc* __restrict__ 
         const TO_GLOBAL T* Xb = X + batch * d->xb;
         TO_GLOBAL T* Cb = C + batch * d->cb;
         const int64_t f0 = ft * ROWS + r;
-        int64_t xo = pdep((uint64_t)f0, d->xf_pos, fx);
+        int64_t xo = pdep_l((uint64_t)f0, XF, 0, fx);
         for (int64_t f = f0; f < ft * ROWS + ROWS; f += rb) {
             T xv[KMAX];
 #pragma unroll
@@ -2700,14 +2775,14 @@ int plan_tensordot(int rank_a, int rank_b, const int32_t* axes_a, const int32_t*
     memcpy(d.xs_pos, x_is_left ? a_sh : b_sh, (size_t)s);
     memcpy(d.ys_pos, x_is_left ? b_sh : a_sh, (size_t)s);
     P.x_is_left = x_is_left;
-    const int esize = entry_type == TO_F64 ? 8 : 4;
     const int64_t K = 1ll << s;
     if (s <= 12) {
         int yc = fy >= 4 ? 16 : 1 << fy;
         int64_t lds = 8 * K;
         if (fx >= 8) {
-            while (yc > 1 && 8 * K + yc * K * esize > kDotLdsLimit) yc >>= 1;
-            lds = 8 * K + yc * K * esize;
+            // the Y slab is held in LDS as the double accumulator type
+            while (yc > 1 && 8 * K + yc * K * 8 > kDotLdsLimit) yc >>= 1;
+            lds = 8 * K + yc * K * 8;
         }
         const int64_t nych = (1ll << fy) / yc;
         if (lds <= kDotLdsLimit && !(s >= 8 && (nych << fx) < 32768)) {

@@ -357,9 +357,10 @@ def choose_variant(g, nb, esize, launch_bound=False):
             return dict(kind="stream", yc=1 << fy, nych=1, lds=lds, stage=0, vec=True,
                         nblocks=int(min(ntiles, MAX_STREAM_BLOCKS)))
     if fx >= 8:
-        while yc > 1 and 8 * K + yc * K * esize > LDS_LIMIT:
+        # the Y slab is held in LDS as the kernel's double accumulator type
+        while yc > 1 and 8 * K + yc * K * 8 > LDS_LIMIT:
             yc //= 2
-        lds = 8 * K + yc * K * esize
+        lds = 8 * K + yc * K * 8
         # outputs of a tile form one contiguous run when the Y-free bits sit below the f
         # bits and one chunk holds all of them: stage that run through LDS
         if STAGE_OUTPUTS and g["shy"] == 0 and g["shf"] == fy and yc == (1 << fy) and fy >= 1 \
