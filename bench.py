@@ -43,6 +43,8 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0  # dense (no sparsity)
+# float work on the bf16 matrix cores: six bf16 products per float product
+X6_PEAK_TFLOPS = BF16_PEAK_TFLOPS / 6
 
 
 def log(*a):
@@ -66,6 +68,8 @@ def kernel_symbol(L, scaled):
             return "k_gemm128<%s, %s, %s, %s>" % (T, S, wide, tr)
         return "k_gemm<%s, %s, %s>" % (T, S, "true" if v & 2 else "false")
     if op == _lib.TO_OP_JOIN_STREAM:
+        if v == _lib.TO_STREAM_WAVE:
+            return "k_join_wave<%s, %s>" % (T, S)
         if v & _lib.TO_STREAM_ANY:
             return "k_join_stream_any<%s, %s>" % (T, S)
         if v & _lib.TO_STREAM_ROWS:
@@ -109,7 +113,7 @@ def roofline_of(name, k, workload=None):
         if name.startswith("k_gemm_x6"):
             # float products on the bf16 matrix cores: each float multiply-add is six bf16
             # ones, so the float flops are priced against the dense bf16 peak / 6
-            peak = BF16_PEAK_TFLOPS / 6
+            peak = X6_PEAK_TFLOPS
             out["peak_basis"] = "dense bf16 peak %.0f TFLOP/s / 6 bf16 products per float product" % BF16_PEAK_TFLOPS
         else:
             peak = FP32_PEAK_TFLOPS if "<float" in name else FP64_PEAK_TFLOPS
@@ -325,6 +329,67 @@ def permute_sweep(rank=26, reps=20):
     print(json.dumps({"permute_sweep": rows, "rank": rank}), flush=True)
 
 
+def gemm_sweep(reps=8):
+    """Standalone split products (k_gemm_x6) on the product shapes of the 4096-slice and
+    rank-28 workloads, each on three operand fills of the same shape: uniform [0, 1) floats,
+    zeros, and floats that bf16 holds exactly (their middle and low bf16 parts are zero).
+    The instruction stream is identical across fills; the time differs only through what
+    the matrix cores draw (the chip lowers its clock under power load), so the zero fill
+    bounds what the kernel's schedule reaches and the uniform fill what random data gets.
+    Each time is the fastest of ``reps`` launches after the first."""
+    import ctypes
+
+    import torch
+
+    from tensororder_amd import _lib, autotune
+    from tensororder_amd.hip_api import HipF32API
+
+    api = HipF32API()
+    # (rows log2, columns log2, shared log2, batch, launch variant, workload)
+    shapes = [(11, 11, 10, 64, 61, "4096-slice 3-CNF: 256-row float-accumulator product"),
+              (11, 10, 12, 64, 31, "4096-slice 3-CNF: wide (double-accumulator) product"),
+              (15, 12, 12, 1, 27, "vertex cover n=200: wide product, columns strided"),
+              (15, 12, 12, 1, 31, "vertex cover n=200: wide product, columns contiguous")]
+    rows = []
+    for mlog, nlog, klog, nb, variant, what in shapes:
+        na, nbb, nc = nb << (mlog + klog), nb << (nlog + klog), nb << (mlog + nlog)
+        a, b, c = api._alloc(4 * na), api._alloc(4 * nbb), api._alloc(4 * nc)
+        recs = autotune._recs(api, 3 * nb)
+        gd = _lib.GemmDesc()
+        gd.a, gd.b, gd.c = a.data_ptr(), b.data_ptr(), c.data_ptr()
+        gd.ab, gd.bb, gd.cb = 1 << (mlog + klog), 1 << (nlog + klog), 1 << (mlog + nlog)
+        tr = bool(variant & 4)
+        gd.ldcm, gd.ldcn = ((1 << nlog), 1) if tr else (1, 1 << mlog)
+        rs = ctypes.sizeof(_lib.ScaleRec)
+        gd.as_, gd.bs, gd.cs = recs.data_ptr(), recs.data_ptr() + nb * rs, recs.data_ptr() + 2 * nb * rs
+        gd.asb = gd.bsb = 1
+        gd.mlog, gd.nlog, gd.klog = mlog, nlog, klog
+        gd.nbatch, gd.block0, gd.nblocks = nb, 0, 256
+        blob = api._alloc(ctypes.sizeof(gd))
+        api._upload(blob, bytes(gd))
+        L = _lib.Launch(op=_lib.TO_OP_GEMM, variant=variant, count=1, grid=256, descs=blob.data_ptr())
+        flops = 2.0 * nb * (1 << (mlog + nlog + klog))
+        for fill in ("uniform", "zero", "bf16-exact"):
+            with torch.cuda.stream(api._stream):
+                for buf, cnt, seed in ((a, na, 1), (b, nbb, 2)):
+                    autotune._fill(api, buf, cnt, 4, seed)
+                    t = buf[: 4 * cnt].view(torch.float32)
+                    if fill == "zero":
+                        t.zero_()
+                    elif fill == "bf16-exact":
+                        t.copy_(t.to(torch.bfloat16).to(torch.float32))
+            api._stream.synchronize()
+            ms = autotune._time(api, [L], reps=reps)
+            tf = flops / (ms * 1e-3) / 1e12
+            rows.append({"m": 1 << mlog, "n": 1 << nlog, "k": 1 << klog, "batch": nb,
+                         "kernel": kernel_symbol(L, True), "workload": what, "fill": fill, "ms": ms,
+                         "TFLOP/s": tf, "frac": tf / X6_PEAK_TFLOPS})
+            log(rows[-1])
+        del a, b, c, recs, blob
+        torch.cuda.empty_cache()
+    print(json.dumps({"gemm_sweep": rows, "peak_TFLOP/s": X6_PEAK_TFLOPS}), flush=True)
+
+
 def timed_steps(exe, steps, warmup, max_s=None):
     """Run ``warmup`` untimed contractions, then time up to ``steps`` contractions (all of
     them unless ``max_s`` is given: then at least one, stopping once ``max_s`` seconds have
@@ -503,6 +568,8 @@ def main():
                     help="only time standalone bit permutations of rank-26 tensors")
     ap.add_argument("--learned-eval", action="store_true",
                     help="only time every candidate plan of the held-out formulas (learned vs reference vs FLOP rule)")
+    ap.add_argument("--gemm-sweep", action="store_true",
+                    help="only time the split products on uniform, zero and bf16-exact operands")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 512 MiB three times (reference for HBM counters)")
     args = ap.parse_args()
@@ -511,6 +578,9 @@ def main():
         return
     if args.permute_sweep:
         permute_sweep()
+        return
+    if args.gemm_sweep:
+        gemm_sweep()
         return
     if args.learned_eval:
         learned_eval(args)
