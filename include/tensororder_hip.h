@@ -198,7 +198,13 @@ typedef struct to_gemm_desc {
     int32_t mlog, nlog, klog;
     int32_t nbatch;
     int32_t block0, nblocks;
-    int32_t pad0, pad1;
+    int32_t lsplit;            /* 64x64 form (variants 0, 2) only: 0, or split K into 2^lsplit
+                                  parts (K >> lsplit >= 16) whose double partial sums go to c
+                                  as double[nbatch << (mlog + nlog)][2^lsplit], output o =
+                                  (batch << (mlog + nlog)) | (m << nlog) | n, for a
+                                  TO_OP_JOIN_FINISH launch (fx = mlog, fy = nlog, nsplit =
+                                  2^lsplit) to add up into C */
+    int32_t pad1;
 } to_gemm_desc;
 
 /* The root scalar of one instance of a batch: stored into its accumulator as

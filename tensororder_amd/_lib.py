@@ -109,7 +109,7 @@ class GemmDesc(ctypes.Structure):
         ("asb", ctypes.c_int64), ("bsb", ctypes.c_int64),
         ("mlog", ctypes.c_int32), ("nlog", ctypes.c_int32), ("klog", ctypes.c_int32),
         ("nbatch", ctypes.c_int32), ("block0", ctypes.c_int32), ("nblocks", ctypes.c_int32),
-        ("pad0", ctypes.c_int32), ("pad1", ctypes.c_int32),
+        ("lsplit", ctypes.c_int32), ("pad1", ctypes.c_int32),
     ]
 
 

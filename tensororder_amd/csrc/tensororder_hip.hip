@@ -590,19 +590,34 @@ __device__ __forceinline__ void join_finish_body(const to_join_desc* __restrict_
     const TO_GLOBAL double* partial = gptr<const double>(d->partial);
     TO_GLOBAL T* C = gptr<T>(d->c);
     const int64_t stride = (int64_t)d->nblocks * kBlock;
-    for (int64_t o = local * kBlock + threadIdx.x; o < nout; o += stride) {
-        double acc = 0.0;
-        for (int sp = 0; sp < nsplit; ++sp) acc += partial[o * nsplit + sp];
-        const T sum = (T)acc;
-        const int64_t batch = o >> (fx + fy);
-        const int64_t f = (o >> fy) & ((1ll << fx) - 1);
-        const int64_t y = o & ((1ll << fy) - 1);
-        C[batch * d->cb + ((f << d->shf) | (y << d->shy))] = sum;
-        if (SCALED) {
-            note_max(d->cs, batch, (float)absval(sum));
-            if (f == 0 && y == 0) {
+    // every lane runs every pass (outputs past the end only idle), so a pass's max can be
+    // reduced across the wave and published with one atomic where the wave's outputs share
+    // a batch item (one per output queued on the record when a finish covers many outputs)
+    for (int64_t o0 = local * kBlock; o0 < nout; o0 += stride) {
+        const int64_t o = o0 + threadIdx.x;
+        const bool valid = o < nout;
+        const int64_t batch = (valid ? o : nout - 1) >> (fx + fy);
+        float m = 0.0f;
+        if (valid) {
+            double acc = 0.0;
+            for (int sp = 0; sp < nsplit; ++sp) acc += partial[o * nsplit + sp];
+            const T sum = (T)acc;
+            const int64_t f = (o >> fy) & ((1ll << fx) - 1);
+            const int64_t y = o & ((1ll << fy) - 1);
+            C[batch * d->cb + ((f << d->shf) | (y << d->shy))] = sum;
+            m = (float)absval(sum);
+            if (SCALED && f == 0 && y == 0) {
                 const int64_t e = op_scale(d->xs, d->xsb, batch).e + op_scale(d->ys, d->ysb, batch).e;
                 note_exp(d->cs, batch, e);
+            }
+        }
+        if (SCALED) {
+            const int64_t b0 = uniform64(batch);
+            if (__all(batch == b0)) {
+                const float w = wave_max(m);
+                if ((threadIdx.x & 63) == 0) note_max(d->cs, b0, w);
+            } else if (valid) {
+                note_max(d->cs, batch, m);
             }
         }
     }
@@ -1623,7 +1638,12 @@ __device__ __forceinline__ void flush_wide(typename Mfma<T>::V (&acc)[N][M], dou
 
 template <typename T, bool SCALED, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict__ descs, int ndesc, const int32_t* __restrict__ bmap) {
-    constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
+    // K slabs of 32: each thread fetches 8 consecutive k of one A row and one B row as
+    // 16-byte loads, one slab ahead (the next slab's loads are in flight while this slab's
+    // MFMAs run); K = 16 fills the slab's upper half with zeros
+    constexpr int BM = 64, BN = 64, BK = 32, PAD = 4, KT = 8;
+    constexpr int VE = 16 / sizeof(T);
+    typedef typename Vec16<T>::type V16;
     __shared__ T As[BK][BM + PAD];
     __shared__ T Bs[BK][BN + PAD];
     typedef typename Mfma<T>::V V;
@@ -1633,19 +1653,26 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
     const int64_t K = 1ll << d->klog;
     const int64_t tiles_m = 1ll << (d->mlog - 6), tiles_n = 1ll << (d->nlog - 6);
     const int64_t per_batch = tiles_m * tiles_n;
-    const int64_t ntiles = (int64_t)d->nbatch * per_batch;
-    const int lrow = tid >> 2, lk = (tid & 3) * 4;
+    // split-K (lsplit > 0): tile t's K range is cut into 2^lsplit parts, one block each,
+    // whose double partial sums go to c (as double[outputs][2^lsplit]) for a
+    // TO_OP_JOIN_FINISH launch to add up, scale-record updates included
+    const int lsplit = d->lsplit;
+    const int64_t KS = K >> lsplit;  // this block's share of K
+    const int64_t ntiles = ((int64_t)d->nbatch * per_batch) << lsplit;
+    const int lrow = tid >> 2, lk = (tid & 3) * KT;
+    const bool kin = lk < KS;  // this thread's 8 k exist (K >= 32, or the lower half of K = 16)
     float mx = 0.0f;        // running max |output| of the batch item mx_batch
     int64_t mx_batch = -1;
     for (int64_t t0 = blockIdx.x - d->block0; t0 < ntiles; t0 += d->nblocks) {
         const int64_t t = xcd_tile(t0, ntiles);
-        const int64_t batch = t / per_batch;
-        const int64_t rem = t - batch * per_batch;
+        const int64_t sp = t & ((1ll << lsplit) - 1);
+        const int64_t batch = (t >> lsplit) / per_batch;
+        const int64_t rem = (t >> lsplit) - batch * per_batch;
         int64_t tm, tn;
         grouped_tile(rem, tiles_m, tiles_n, tm, tn);
-        if (SCALED) publish_max_on_change(d->cs, mx_batch, batch, mx, lane);
-        const TO_GLOBAL T* A = gptr<const T>(d->a) + batch * d->ab + (tm * BM + lrow) * K + lk;
-        const TO_GLOBAL T* B = gptr<const T>(d->b) + batch * d->bb + (tn * BN + lrow) * K + lk;
+        if (SCALED && lsplit == 0) publish_max_on_change(d->cs, mx_batch, batch, mx, lane);
+        const TO_GLOBAL T* A = gptr<const T>(d->a) + batch * d->ab + (tm * BM + lrow) * K + sp * KS + lk;
+        const TO_GLOBAL T* B = gptr<const T>(d->b) + batch * d->bb + (tn * BN + lrow) * K + sp * KS + lk;
         T sa = T(1), sb = T(1);
         int64_t ea = 0, eb = 0;
         if (SCALED) {
@@ -1663,21 +1690,27 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
                     acc[i][j][v] = T(0);
                     wide[i][j][v] = 0.0;
                 }
+        V16 ra[KT / VE], rb[KT / VE];
+        auto fetch = [&](int64_t k0) {
+#pragma unroll
+            for (int j = 0; j < KT / VE; ++j) {
+                ra[j] = kin ? *reinterpret_cast<const TO_GLOBAL V16*>(A + k0 + j * VE) : V16{};
+                rb[j] = kin ? *reinterpret_cast<const TO_GLOBAL V16*>(B + k0 + j * VE) : V16{};
+            }
+        };
+        fetch(0);
         int64_t step = 0;
-        for (int64_t k0 = 0; k0 < K; k0 += BK, ++step) {
-            T ra[4], rb[4];
+        for (int64_t k0 = 0; k0 < KS; k0 += BK, ++step) {
+            __syncthreads();  // the previous slab's fragments have been read
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                ra[j] = A[k0 + j] * sa;
-                rb[j] = B[k0 + j] * sb;
-            }
-            __syncthreads();
+            for (int j = 0; j < KT / VE; ++j)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                As[lk + j][lrow] = ra[j];
-                Bs[lk + j][lrow] = rb[j];
-            }
+                for (int e = 0; e < VE; ++e) {
+                    As[lk + j * VE + e][lrow] = ra[j][e] * sa;
+                    Bs[lk + j * VE + e][lrow] = rb[j][e] * sb;
+                }
             __syncthreads();
+            if (k0 + BK < KS) fetch(k0 + BK);
 #pragma unroll
             for (int kk = 0; kk < BK / 4; ++kk) {
                 const int kr = kk * 4 + (lane >> 4);
@@ -1691,7 +1724,24 @@ __global__ __launch_bounds__(kBlock) void k_gemm(const to_gemm_desc* __restrict_
 #pragma unroll
                     for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
             }
-            if (kWideAccum<T, WIDE> && ((step & 3) == 3 || k0 + BK >= K)) flush_wide<T, 2, 2>(acc, wide);
+            // partial sums of 64 terms go into the double accumulators
+            if (kWideAccum<T, WIDE> && ((step & 1) == 1 || k0 + BK >= KS)) flush_wide<T, 2, 2>(acc, wide);
+        }
+        if (lsplit > 0) {
+            TO_GLOBAL double* P = gptr<double>(d->c);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int64_t n = tn * BN + wn * 32 + j * 16 + (lane & 15);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int64_t m = tm * BM + wm * 32 + i * 16 + Mfma<T>::row(lane, v);
+                        const int64_t o = (batch << (d->mlog + d->nlog)) | (m << d->nlog) | n;
+                        P[(o << lsplit) + sp] = kWideAccum<T, WIDE> ? wide[i][j][v] : (double)acc[i][j][v];
+                    }
+                }
+            continue;
         }
         TO_GLOBAL T* C = gptr<T>(d->c) + batch * d->cb;
 #pragma unroll

@@ -33,6 +33,10 @@ LDS_LIMIT = 64 * 1024
 MAX_STREAM_BLOCKS = 4096
 # one-wave streamed joins (many-instance batch): at most this many products per lane
 WAVE_MAX_LANE_PRODUCTS = 64
+# products of the many-instance batch with fewer 64x64 tiles than this split K (in parts of
+# at least 2^GEMM_SPLIT_MIN_KLOG) to reach it
+GEMM_SPLIT_BLOCKS = 256
+GEMM_SPLIT_MIN_KLOG = 10
 # the many-instance batch takes the row form for joins of at least this many output bits
 ROWS_MIN_BITS_BATCH = 20
 
@@ -325,10 +329,23 @@ def choose_variant(g, nb, esize, launch_bound=False):
         # float outputs whose columns are contiguous (shy == 0) are written 16 bytes at a
         # time from a transposed product
         tr = big and esize == 4 and g["shy"] == 0
-        return dict(kind="gemm", tile=(1 if big else 0) | (2 if wide else 0) | (4 if tr else 0) | (8 if db else 0)
-                    | (16 if x6 else 0) | (32 if tall else 0),
-                    nblocks=int(min(tiles, cap)),
-                    perm_x=not _canonical(g["xf"], g["xs"]), perm_y=not _canonical(g["yf"], g["ys"]))
+        out = dict(kind="gemm", tile=(1 if big else 0) | (2 if wide else 0) | (4 if tr else 0) | (8 if db else 0)
+                   | (16 if x6 else 0) | (32 if tall else 0),
+                   nblocks=int(min(tiles, cap)),
+                   perm_x=not _canonical(g["xf"], g["xs"]), perm_y=not _canonical(g["yf"], g["ys"]))
+        if launch_bound and not big:
+            # few 64x64 tiles over a long K (a batch step's products run one tile per block
+            # through all of K): split K into parts of >= 2^GEMM_SPLIT_MIN_KLOG until the
+            # product has GEMM_SPLIT_BLOCKS blocks; a finishing launch adds the parts
+            lsplit = 0
+            while (tiles << lsplit) < GEMM_SPLIT_BLOCKS and s - lsplit > GEMM_SPLIT_MIN_KLOG:
+                lsplit += 1
+            if lsplit:
+                nout = nb << (fx + fy)
+                out.update(lsplit=lsplit, nsplit=1 << lsplit, nout=nout,
+                           nblocks=int(min(tiles << lsplit, cap)),
+                           fin_blocks=int(min(-(-nout // 256), MAX_STREAM_BLOCKS)))
+        return out
     # the many-instance batch: a join whose outputs need at most WAVE_MAX_LANE_PRODUCTS
     # products per lane of one wave runs as one wave, four joins to a block (the kernel
     # holds the 2^s shared offsets one per lane: s <= 6)
@@ -620,7 +637,7 @@ def _temporaries(arena, n, nodes, W, esize, partial, keep=False):
     caller to release (``keep``) when other joins run beside it."""
     v = n.geom["var"]
     blocks = []
-    if v["kind"] == "splitk":
+    if v["kind"] == "splitk" or v.get("lsplit"):
         blocks.append(("partial", arena.alloc(v["nout"] * v["nsplit"] * 8)))  # double partial sums
     if v["kind"] in ("gemm", "splitk"):
         g = n.geom
@@ -783,8 +800,13 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp, launch_bound=
         gd.mlog, gd.nlog, gd.klog = g["fx"], g["fy"], g["s"]
         gd.nbatch = nb
         gd.block0, gd.nblocks = 0, v["nblocks"]
+        if v.get("lsplit"):
+            # split K: the parts' partial sums go to scratch, a finishing launch writes C
+            assert not v["tile"] & 1
+            gd.c, gd.lsplit = abase + scratch["partial"][0], v["lsplit"]
         out.append(("gemm", _lib.TO_OP_GEMM, v["tile"], 0, gd, v["nblocks"]))
-        return out
+        if not v.get("lsplit"):
+            return out
     d = JoinDesc()
     d.x, d.y, d.c = X.ptr, Y.ptr, n.ptr
     d.xb = X.bstride if n.batched else 0
@@ -808,6 +830,12 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp, launch_bound=
     d.block0 = 0
     d.xf_pos, d.xs_pos = _pos_array(xpos[0]), _pos_array(xpos[1])
     d.yf_pos, d.ys_pos = _pos_array(ypos[0]), _pos_array(ypos[1])
+    if v["kind"] == "gemm":  # the finishing launch of a split-K product
+        d.partial = abase + scratch["partial"][0]
+        d.nsplit = v["nsplit"]
+        d.nblocks = v["fin_blocks"]
+        out.append(("finish", _lib.TO_OP_JOIN_FINISH, 0, 0, d, v["fin_blocks"]))
+        return out
     if v["kind"] == "stream":
         d.nblocks, d.nych, d.nsplit = v["nblocks"], v["nych"], 1
         d.lnych = v["nych"].bit_length() - 1

@@ -84,6 +84,8 @@ class Emulator:
         self.dt = np.float32 if self.scaled else np.float64
         self.maps_checked = 0
         self.wave_joins = 0
+        self.gemm_partials = set()
+        self.split_gemms = 0
 
     def recs(self, addr, count):
         raw = self.heap.view(addr, np.uint8, 16 * count).reshape(count, 16)
@@ -192,7 +194,32 @@ class Emulator:
 
     def op_4(self, L):
         for i in range(L.count):
-            self._join(self.heap.read_struct(_lib.JoinDesc, L.descs + i * ctypes.sizeof(_lib.JoinDesc)))
+            d = self.heap.read_struct(_lib.JoinDesc, L.descs + i * ctypes.sizeof(_lib.JoinDesc))
+            if d.partial in self.gemm_partials:
+                self._finish_from_partials(d)
+            else:
+                self._join(d)
+
+    def _finish_from_partials(self, d):
+        """TO_OP_JOIN_FINISH over the partial sums a split-K product wrote: output o =
+        (batch << (fx + fy)) | (f << fy) | y adds partial[o * nsplit + sp] over sp."""
+        fx, fy, dt = d.fx, d.fy, self.dt
+        nout = d.nbatch << (fx + fy)
+        P = self.heap.view(d.partial, np.float64, nout * d.nsplit).reshape(nout, d.nsplit)
+        sums = P.sum(axis=1).astype(dt).reshape(d.nbatch, 1 << fx, 1 << fy)
+        f = np.arange(1 << fx, dtype=np.int64)
+        y = np.arange(1 << fy, dtype=np.int64)
+        cidx = (f[:, None] << d.shf) | (y[None, :] << d.shy)
+        esz = np.dtype(dt).itemsize
+        for b in range(d.nbatch):
+            C = self.heap.view(d.c + b * d.cb * esz, dt, int(cidx.max() + 1))
+            C[cidx] = sums[b]
+            if self.scaled:
+                e0, m0 = self.get_rec(d.xs, b * d.xsb)
+                e1, m1 = self.get_rec(d.ys, b * d.ysb)
+                m = float(np.max(np.abs(sums[b])))
+                self.set_rec(d.cs, b, exp=e0 + _scale_exp(m0) + e1 + _scale_exp(m1),
+                             maxbits=int(np.array([m], np.float32).view(np.uint32)[0]))
 
     def op_5(self, L):
         raw = self.recs(L.aux0, L.iaux0)
@@ -242,6 +269,24 @@ class Emulator:
             m = np.arange(M, dtype=np.int64)
             nn = np.arange(N, dtype=np.int64)
             cidx = m[:, None] * d.ldcm + nn[None, :] * d.ldcn
+            if d.lsplit:
+                # split K: the parts' partial sums, double[o][2^lsplit] with
+                # o = (b << (mlog + nlog)) | (m << nlog) | n
+                S = 1 << d.lsplit
+                P = self.heap.view(d.c, np.float64, (d.nbatch << (d.mlog + d.nlog)) * S)
+                P = P.reshape(d.nbatch, M, N, S)
+                for b in range(d.nbatch):
+                    A = self.heap.view(d.a + b * d.ab * esz, self.dt, M * K).reshape(M, K).astype(np.float64)
+                    B = self.heap.view(d.b + b * d.bb * esz, self.dt, N * K).reshape(N, K).astype(np.float64)
+                    if self.scaled:
+                        A = A * 2.0 ** -_scale_exp(self.get_rec(d.as_, b * d.asb)[1])
+                        B = B * 2.0 ** -_scale_exp(self.get_rec(d.bs, b * d.bsb)[1])
+                    for sp in range(S):
+                        ks = slice(sp * (K // S), (sp + 1) * (K // S))
+                        P[b, :, :, sp] = A[:, ks] @ B[:, ks].T
+                self.gemm_partials.add(d.c)
+                self.split_gemms += 1
+                continue
             for b in range(d.nbatch):
                 A = self.heap.view(d.a + b * d.ab * esz, self.dt, M * K).reshape(M, K).astype(np.float64)
                 B = self.heap.view(d.b + b * d.bb * esz, self.dt, N * K).reshape(N, K).astype(np.float64)
@@ -398,4 +443,5 @@ def emulate_batch(instances, entry_type, budget=None):
             out.append(math.ldexp(acc.mant * fm, acc.exp + fe))
     progs[0].maps_checked = emu.maps_checked
     progs[0].wave_joins = emu.wave_joins
+    progs[0].split_gemms = emu.split_gemms
     return out, (progs[0] if budget is None else progs)

@@ -205,6 +205,30 @@ def _gemm_descs(prog):
     return out
 
 
+@pytest.mark.parametrize("et,tol", [(_lib.TO_F64, 1e-11), (_lib.TO_F32S, 2e-5)])
+def test_batch_split_k_products_emulated(et, tol, monkeypatch):
+    # products of a batch step with few 64x64 tiles over a long K are split into K parts
+    # whose partial sums a finishing launch adds (the emulator writes and sums the parts in
+    # the documented layout); every count against the oracle.  Parts of >= 128 terms here,
+    # so that products small enough to emulate are split
+    monkeypatch.setattr(schedule, "GEMM_SPLIT_MIN_KLOG", 7)
+    instances, wants = [], []
+    for i, (ra, rb, s) in enumerate(((16, 16, 10), (15, 16, 9), (17, 16, 10), (14, 14, 8))):
+        plan = _two_tensor_plan(ra, rb, s, seed=50 + i)
+        wants.append(numpy_oracle.contract_sliced(plan_io.plan_to_dict(plan)))
+        tree, tables, G, total = emulator.plan_tables(plan, emulator.HostOps)
+        instances.append((tree, tables))
+    got, prog = emulator.emulate_batch(instances, et)
+    assert prog.split_gemms >= 3
+    gemms = [L for L in prog.launches if L.op == _lib.TO_OP_GEMM]
+    finishes = [L for L in prog.launches if L.op == _lib.TO_OP_JOIN_FINISH]
+    assert gemms and finishes
+    for L in gemms:
+        assert L.variant in (0, 2)  # the 64x64 form, the only one that reads lsplit
+    for g, w in zip(got, wants):
+        assert g == pytest.approx(w, rel=tol, abs=tol * 1e-3 * abs(w))
+
+
 @pytest.mark.parametrize("et", [_lib.TO_F64, _lib.TO_F32S])
 def test_gemm_descriptors_fit_their_tiles(et):
     # every product the schedule hands to the MFMA kernels satisfies the tile shape the

@@ -340,6 +340,79 @@ def test_one_wave_stream_joins(hip, hip32, et):
 
 
 @pytest.mark.parametrize("et", ["f64", "f32s"])
+def test_split_k_product_and_finish(hip, hip32, et):
+    # the 64x64 product with K split into 2^lsplit parts (double partial sums in the
+    # documented layout) and the finishing launch that adds them, for both output layouts,
+    # against the float64 product; FP32-scaled outputs carry their exponent and max
+    import ctypes
+
+    import torch
+
+    from tensororder_amd import _lib
+
+    api = hip if et == "f64" else hip32
+    esize = api.get_entry_size()
+    dt = np.float64 if esize == 8 else np.float32
+    rng = np.random.default_rng(13)
+    dev = torch.device("cuda", 0)
+    rsz = ctypes.sizeof(_lib.ScaleRec)
+    for mlog, nlog, klog, nb, lsplit in ((6, 6, 10, 1, 3), (7, 6, 9, 2, 2), (6, 7, 12, 1, 5)):
+        for variant in ((0, 2) if esize == 4 else (0,)):
+            for cols_contiguous in (True, False):
+                _say("case split-K", et, mlog, nlog, klog, nb, lsplit, variant, cols_contiguous)
+                M, N, K, S = 1 << mlog, 1 << nlog, 1 << klog, 1 << lsplit
+                a = rng.uniform(0.5, 2.0, (nb, M, K)).astype(dt)
+                b = rng.uniform(0.5, 2.0, (nb, N, K)).astype(dt)
+                want = np.einsum("bmk,bnk->bmn", a.astype(np.float64), b.astype(np.float64))
+                ta, tb = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+                tc = torch.full((nb, M * N), float("nan"), dtype=ta.dtype, device=dev)
+                tp = torch.full((nb * M * N * S,), float("nan"), dtype=torch.float64, device=dev)
+                recs = None
+                if esize == 4:
+                    rs = np.zeros((3 * nb, 2), dtype=np.int64)
+                    for i in range(nb):
+                        rs[i] = (2, int(np.float32(np.abs(a[i]).max()).view(np.uint32)))
+                        rs[nb + i] = (-1, int(np.float32(np.abs(b[i]).max()).view(np.uint32)))
+                    recs = torch.from_numpy(rs).to(dev)
+                gd = _lib.GemmDesc()
+                gd.a, gd.b, gd.c = ta.data_ptr(), tb.data_ptr(), tp.data_ptr()
+                gd.ab, gd.bb, gd.cb = M * K, N * K, M * N
+                gd.ldcm, gd.ldcn = (N, 1) if cols_contiguous else (1, M)
+                gd.mlog, gd.nlog, gd.klog = mlog, nlog, klog
+                gd.nbatch, gd.block0, gd.nblocks, gd.lsplit = nb, 0, 5, lsplit
+                fd = _lib.JoinDesc()
+                fd.c, fd.cb, fd.partial, fd.nsplit = tc.data_ptr(), M * N, tp.data_ptr(), S
+                fd.fx, fd.fy, fd.nbatch, fd.nblocks = mlog, nlog, nb, 3
+                fd.shf, fd.shy = (nlog, 0) if cols_contiguous else (0, mlog)
+                if recs is not None:
+                    gd.as_, gd.bs, gd.cs = recs.data_ptr(), recs.data_ptr() + nb * rsz, recs.data_ptr() + 2 * nb * rsz
+                    gd.asb = gd.bsb = 1
+                    fd.xs, fd.ys, fd.cs = gd.as_, gd.bs, gd.cs
+                    fd.xsb = fd.ysb = 1
+                blob = torch.frombuffer(bytearray(bytes(gd) + bytes(fd)), dtype=torch.uint8).to(dev)
+                torch.cuda.synchronize()
+                launches = [_lib.Launch(op=_lib.TO_OP_GEMM, variant=variant, count=1, grid=gd.nblocks,
+                                        descs=blob.data_ptr()),
+                            _lib.Launch(op=_lib.TO_OP_JOIN_FINISH, variant=0, count=1, grid=fd.nblocks,
+                                        descs=blob.data_ptr() + ctypes.sizeof(gd))]
+                arr = _lib.launch_array(launches)
+                _lib.check(api._lib.to_run(ctypes.addressof(arr), 2, api._entry_type,
+                                           ctypes.c_void_p(api._stream.cuda_stream)), "to_run")
+                api._stream.synchronize()
+                c = tc.cpu().numpy().astype(np.float64)
+                c = c.reshape(nb, M, N) if cols_contiguous else c.reshape(nb, N, M).transpose(0, 2, 1)
+                if esize == 8:
+                    np.testing.assert_allclose(c, want, rtol=1e-12)
+                    continue
+                r = recs.cpu().numpy()
+                for i in range(nb):
+                    # stored * 2^exp of the output record; the operands carry 2^2 and 2^-1
+                    np.testing.assert_allclose(np.ldexp(c[i], int(r[2 * nb + i, 0])), np.ldexp(want[i], 1),
+                                               rtol=2e-6)
+                    assert int(r[2 * nb + i, 1]) & 0xFFFFFFFF == int(np.float32(np.abs(c[i]).max()).view(np.uint32))
+
+
+@pytest.mark.parametrize("et", ["f64", "f32s"])
 def test_gemm_every_tile_variant(hip, hip32, et):
     # every MFMA product kernel (64x64, 128x128, 128x64 wide, 128x128 one-block-per-CU
     # double-buffered, bf16 three-way split; plain and transposed) on the same operands,
