@@ -23,9 +23,6 @@ KIND_ORDER = schedule.KIND_ORDER
 # a grouped launch over at least this many descriptors also carries a block -> descriptor
 # map (int32 per block, in aux1), so each block finds its descriptor in one load
 MAP_MIN_DESCS = 8
-# streamed-join forms that keep a launch of their own in a step instead of joining the
-# mixed (TO_STREAM_ANY) launch
-OWN_LAUNCH_FORMS = frozenset()
 
 
 class BatchProgram:
@@ -186,8 +183,7 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True, per=None, a
                     n, nodes, 1, esize, scaled, abase, scratch.get(n.idx, {}), recp, launch_bound=True):
                 # the streamed joins of one step share one launch whatever their form
                 # (each descriptor names its own); the step is bound by launch count
-                if op == _lib.TO_OP_JOIN_STREAM and variant != _lib.TO_STREAM_WAVE \
-                        and variant not in OWN_LAUNCH_FORMS:
+                if op == _lib.TO_OP_JOIN_STREAM and variant != _lib.TO_STREAM_WAVE:
                     schedule.check_stream_form(st.kvar)
                     variant = _lib.TO_STREAM_ANY
                 key = (side, KIND_ORDER[kind], op, variant)
