@@ -37,6 +37,8 @@ WAVE_MAX_LANE_PRODUCTS = 64
 # at least 2^GEMM_SPLIT_MIN_KLOG) to reach it
 GEMM_SPLIT_BLOCKS = 256
 GEMM_SPLIT_MIN_KLOG = 10
+# split-K joins of the many-instance batch: each split walks >= 2^this steps of 256
+SPLITK_BATCH_MIN_LOG_STEPS = 4
 # the many-instance batch takes the row form for joins of at least this many output bits
 ROWS_MIN_BITS_BATCH = 20
 
@@ -406,6 +408,10 @@ def choose_variant(g, nb, esize, launch_bound=False):
     nsplit = 1
     if s >= 8:
         nsplit = 1 << int(min(s - 8, max(0, math.floor(math.log2(max(1, 4096 // nout))))))
+        if launch_bound:
+            # a batch step holds many split-K joins: each split walks at least
+            # 2^SPLITK_BATCH_MIN_LOG_STEPS steps of 256 shared indices
+            nsplit = min(nsplit, 1 << max(0, s - 8 - SPLITK_BATCH_MIN_LOG_STEPS))
     # finishing: one thread per output, or one block per output when it has many partials
     fin_wide = nsplit >= 32
     fin_blocks = min(nout, MAX_STREAM_BLOCKS) if fin_wide else min(-(-nout // 256), MAX_STREAM_BLOCKS)
