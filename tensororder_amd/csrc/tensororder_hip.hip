@@ -1171,12 +1171,21 @@ __global__ __launch_bounds__(kBlock) void k_join_wave(const to_join_desc* __rest
         kxl = (uint32_t)pdep(lane, d->xs_pos, s);
         kyl = (uint32_t)pdep(lane, d->ys_pos, s);
     }
-    // the offsets of this lane's own output bits 0-5
+    // Fewer than 64 outputs: 2^lsub lanes share an output, each summing a contiguous share
+    // of K (the share's high k bits are the lane's q, its low bits walk the loop), and the
+    // shares are added across those lanes.  Lane = (output within the pass << lsub) | q.
+    const int lsub = nbits < 6 ? min(s, 6 - nbits) : 0;
+    const int q = lane & ((1 << lsub) - 1), ol = lane >> lsub;
+    const int KL = K >> lsub;                 // terms per lane
+    const int obits = 6 - lsub;               // output bits per pass
+    const uint32_t xq = (uint32_t)__shfl((int)kxl, q << (s - lsub), 64);
+    const uint32_t yq = (uint32_t)__shfl((int)kyl, q << (s - lsub), 64);
+    // the offsets of this lane's own output bits (the low obits of the output index)
     uint32_t xlane = 0, ylane = 0, clane = 0;
-    const int nlow = nbits < 6 ? nbits : 6;
+    const int nlow = nbits < obits ? nbits : obits;
     for (int t = 0; t < nlow; ++t) {
         const uint32_t a = lane_value(cx, t), b = lane_value(cy, t), c = lane_value(cc, t);
-        if ((lane >> t) & 1) {
+        if ((ol >> t) & 1) {
             xlane |= a;
             ylane |= b;
             clane |= c;
@@ -1197,22 +1206,28 @@ __global__ __launch_bounds__(kBlock) void k_join_wave(const to_join_desc* __rest
         const auto Yb = Y + b * d->yb;
         const auto Cb = C + b * d->cb;
         float m = 0.0f;
-        for (int j = 0; (j << 6) < nout; ++j) {
-            // output bits 6.. are those of j, the same in every lane
+        for (int j = 0; (j << obits) < nout; ++j) {
+            // output bits obits.. are those of j, the same in every lane
             uint32_t xj = 0, yj = 0, cj = 0;
-            for (int t = 6; t < nbits; ++t) {
-                if ((j >> (t - 6)) & 1) {
+            for (int t = obits; t < nbits; ++t) {
+                if ((j >> (t - obits)) & 1) {
                     xj |= lane_value(cx, t);
                     yj |= lane_value(cy, t);
                     cj |= lane_value(cc, t);
                 }
             }
-            if ((j << 6) + lane < nout) {
-                const auto Xp = Xb + (xlane | xj);
-                const auto Yp = Yb + (ylane | yj);
-                Acc<T> acc = 0.0;
-                for (int k = 0; k < K; ++k)
+            const bool valid = (j << obits) + ol < nout;
+            Acc<T> acc = 0.0;
+            if (valid) {
+                const auto Xp = Xb + (xlane | xj | xq);
+                const auto Yp = Yb + (ylane | yj | yq);
+#pragma unroll 4
+                for (int k = 0; k < KL; ++k)
                     acc += (Acc<T>)(Xp[lane_value(kxl, k)] * T(sx)) * (Acc<T>)(Yp[lane_value(kyl, k)] * T(sy));
+            }
+            // every lane takes part in the sums across an output's lanes
+            for (int off = 1; off < (1 << lsub); off <<= 1) acc += __shfl_xor(acc, off, 64);
+            if (valid && q == 0) {
                 const T v = (T)acc;
                 Cb[clane | cj] = v;
                 if (SCALED) m = fmaxf(m, (float)absval(v));

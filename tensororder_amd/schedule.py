@@ -32,7 +32,7 @@ KIND_ORDER = {"permute": 0, "stream": 1, "splitk": 1, "gemm": 1, "finish": 2}
 LDS_LIMIT = 64 * 1024
 MAX_STREAM_BLOCKS = 4096
 # one-wave streamed joins (many-instance batch): at most this many products per lane
-WAVE_MAX_LANE_PRODUCTS = 64
+WAVE_MAX_LANE_PRODUCTS = 32
 # products of the many-instance batch with fewer 64x64 tiles than this split K (in parts of
 # at least 2^GEMM_SPLIT_MIN_KLOG) to reach it
 GEMM_SPLIT_BLOCKS = 256
