@@ -859,6 +859,17 @@ def join_records(n, nodes, W, esize, scaled, abase, scratch, recp, launch_bound=
     return out
 
 
+def block_cost(st):
+    """Rough work of one block of a descriptor, to start the longest blocks of a grouped
+    launch first: its shared-index loop length times the tiles it walks."""
+    if isinstance(st, _lib.GemmDesc):
+        tiles = ((st.nbatch << max(0, st.mlog + st.nlog - 12)) << st.lsplit) // max(1, st.nblocks)
+        return (1 << (st.klog - st.lsplit)) * max(1, tiles)
+    if isinstance(st, JoinDesc):
+        return (1 << st.s) * max(1, (st.nbatch << max(0, st.fx + st.fy - 8)) // max(1, st.nblocks))
+    return 0
+
+
 def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regroup=True, group_levels=True):
     """Build a Program for batch size W.
 
@@ -1029,7 +1040,8 @@ def compile_program(tree, tables, num_groups, entry_type, W, device_alloc, regro
                 key = (KIND_ORDER[kind], op, variant)
                 groups.setdefault(key, []).append((kind, st, nblk, lds, n.idx, work))
         for key in sorted(groups):
-            grp = groups[key]
+            # the joins whose blocks run longest start first (blocks start in block order)
+            grp = sorted(groups[key], key=lambda g: -block_cost(g[1]))
             block0, lds, wb, wf, structs = 0, 0, 0, 0, []
             for kind, st, nblk, l, idx, work in grp:
                 st.block0 = block0
