@@ -2136,14 +2136,15 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
         };
         // The six products of LDS buffer `buf`, the large one last (a float accumulator
         // that takes the small terms first rounds less): (l,h) (m,h) (m,m) (h,m) (h,l) (h,h).
-        // Consecutive products share one operand plane, so each step reads the fragments
-        // of one new plane (7 plane reads for 6 products) while the previous product's
-        // MFMAs run; scheduling barriers keep the products in order, so two planes of
-        // fragments (plus the one being read) are live, not all six.
+        // Consecutive products share one operand plane, so each product is preceded by the
+        // read of at most one new plane, issued while the previous product's MFMAs run: six
+        // plane reads for six products (B's high plane stays in registers from the first
+        // product to the last).  Scheduling barriers keep the products in order, so at most
+        // four planes of fragments are live, not all six.
         auto step = [&](int buf, bool fresh, int64_t k2) {
             const int nbuf = buf ^ 1;
             const unsigned char* S = smem + buf * BUF;
-            U4 fa[4], fb[4], fn[4];
+            U4 fa[4], fb[4], fn[4], fh[4];
             auto read_a = [&](U4 (&f)[4], int plane) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) f[i] = *reinterpret_cast<const U4*>(S + plane * PA + frag_a + i * 256);
@@ -2183,20 +2184,19 @@ __device__ __forceinline__ void gemm_x6_body(const to_gemm_desc* __restrict__ de
             pipe(1, nbuf, k2);
             x6_interleave<x6_groups(1, NT, TJ), 4 * TJ>();
             __builtin_amdgcn_sched_barrier(0);
-            read_a(fb, 0);   // A h (next; B h is done)
+            read_a(fh, 0);   // A h (next; B h stays for the last product)
             product(fn, fa);                       // (m,m)
             pipe(2, nbuf, k2);
             x6_interleave<x6_groups(2, NT, TJ), 4 * TJ>();
             __builtin_amdgcn_sched_barrier(0);
-            read_b(fn, 2);   // B l (next; A m is done)
-            product(fb, fa);                       // (h,m)
+            read_b(fn, 2);   // B l (next; A m is done): the sixth and last plane read
+            product(fh, fa);                       // (h,m)
             pipe(3, nbuf, k2);
             x6_interleave<x6_groups(3, NT, TJ), 4 * TJ>();
             __builtin_amdgcn_sched_barrier(0);
-            read_b(fa, 0);   // B h again (next; B m is done)
-            product(fb, fn);                       // (h,l)
+            product(fh, fn);                       // (h,l)
             __builtin_amdgcn_sched_barrier(0);
-            product(fb, fa);                       // (h,h)
+            product(fh, fb);                       // (h,h)
         };
 #pragma unroll
         for (int v = 0; v < NT; ++v) fetch_vec(v, 0);
