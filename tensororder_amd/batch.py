@@ -23,8 +23,6 @@ KIND_ORDER = schedule.KIND_ORDER
 # a grouped launch over at least this many descriptors also carries a block -> descriptor
 # map (int32 per block, in aux1), so each block finds its descriptor in one load
 MAP_MIN_DESCS = 8
-# order each grouped launch's descriptors by the work of one of their blocks, longest first
-LONGEST_FIRST = True
 
 
 class BatchProgram:
@@ -236,11 +234,9 @@ def compile_batch(instances, entry_type, device_alloc, regroup=True, per=None, a
     for k in range(nsteps):
         step_start = len(pending)
         for key in sorted(steps.get(k, {})):
-            group = steps[k][key]
-            if LONGEST_FIRST:
-                # blocks start in block order: the joins whose blocks run longest go first,
-                # so that they do not start last and hold the step open alone
-                group = sorted(group, key=lambda g: -schedule.block_cost(g[0]))
+            # blocks start in block order: the joins whose blocks run longest go first, so
+            # that they do not start last and hold the step open alone
+            group = sorted(steps[k][key], key=lambda g: -schedule.block_cost(g[0]))
             structs, block0, lds, wb, wf = [], 0, 0, 0, 0
             waves = key[2] == _lib.TO_OP_JOIN_STREAM and key[3] == _lib.TO_STREAM_WAVE
             for j, (st, nblk, l, kind, i, work) in enumerate(group):
