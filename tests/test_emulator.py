@@ -408,3 +408,52 @@ def test_device_leaf_build_matches_host_tables(et):
             np.testing.assert_array_equal(got, rows, err_msg=name)
         n += 1
     assert n > 100
+
+
+def _geom(fx, fy, s, yinner=True):
+    return dict(fx=fx, fy=fy, s=s, shf=fy if yinner else 0, shy=0 if yinner else fx,
+                xf=list(range(s, s + fx)), xs=list(range(s)), yf=list(range(s, s + fy)), ys=list(range(s)),
+                x_is_left=True)
+
+
+@pytest.mark.parametrize("esize", [4, 8])
+def test_batch_launch_shapes(esize):
+    # the many-instance batch's launch-bound choices stay within what their kernels take:
+    # one-wave joins only for s <= 6 and <= WAVE_MAX_LANE_PRODUCTS products per lane;
+    # split-K products only on the 64x64 form, with parts of >= 2^GEMM_SPLIT_MIN_KLOG terms;
+    # split-K joins with >= 2^SPLITK_BATCH_MIN_LOG_STEPS steps of 256 per split
+    waves = splits = 0
+    for fx in range(0, 15):
+        for fy in range(0, 12):
+            for s in range(0, 16):
+                v = schedule.choose_variant(_geom(fx, fy, s), 1, esize, launch_bound=True)
+                if v.get("wave"):
+                    waves += 1
+                    assert s <= 6 and (-(-(1 << (fx + fy)) // 64) << s) <= schedule.WAVE_MAX_LANE_PRODUCTS
+                    assert schedule.stream_variant(v, None) == _lib.TO_STREAM_WAVE
+                if v["kind"] == "gemm" and v.get("lsplit"):
+                    splits += 1
+                    assert not v["tile"] & 1 and s - v["lsplit"] >= schedule.GEMM_SPLIT_MIN_KLOG
+                    tiles = 1 << (fx - 6 + fy - 6)
+                    assert (tiles << (v["lsplit"] - 1)) < schedule.GEMM_SPLIT_BLOCKS
+                    assert v["nout"] == 1 << (fx + fy) and v["nsplit"] == 1 << v["lsplit"]
+                if v["kind"] == "splitk" and s >= 8:
+                    assert v["nsplit"] <= 1 << max(0, s - 8 - schedule.SPLITK_BATCH_MIN_LOG_STEPS)
+                # outside the batch neither form is chosen
+                w = schedule.choose_variant(_geom(fx, fy, s), 1, esize)
+                assert not w.get("wave") and not w.get("lsplit")
+    assert waves > 0 and splits > 0
+
+
+def test_block_cost_orders_longest_first():
+    # a grouped launch's descriptors go longest block first, and the block ranges stay
+    # contiguous and increasing in descriptor order (what the kernels' search assumes)
+    short, long_ = _lib.JoinDesc(), _lib.JoinDesc()
+    short.fx, short.fy, short.s, short.nbatch, short.nblocks = 9, 3, 1, 1, 8
+    long_.fx, long_.fy, long_.s, long_.nbatch, long_.nblocks = 9, 3, 6, 1, 8
+    assert schedule.block_cost(long_) > schedule.block_cost(short)
+    g1, g2 = _lib.GemmDesc(), _lib.GemmDesc()
+    g1.mlog, g1.nlog, g1.klog, g1.nbatch, g1.nblocks = 6, 6, 12, 1, 1
+    g2.mlog, g2.nlog, g2.klog, g2.nbatch, g2.nblocks, g2.lsplit = 6, 6, 12, 1, 8, 3
+    assert schedule.block_cost(g1) > schedule.block_cost(g2)
+    assert schedule.block_cost(_lib.PermDesc()) == 0
