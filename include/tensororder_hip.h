@@ -230,7 +230,8 @@ typedef struct to_launch {
                                         LEAF_BUILD) aux1 is 0 or a device int32[grid] map of
                                         each block to its descriptor (the last one whose
                                         block0 is at or below the block), read instead of a
-                                        binary search over block0 */
+                                        binary search over block0 (TO_STREAM_WAVE launches
+                                        ignore it: block and wave fix the descriptor) */
     int64_t iaux0, iaux1;
 } to_launch;
 
