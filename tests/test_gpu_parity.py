@@ -435,6 +435,14 @@ def test_gemm_every_tile_variant(hip, hip32, et):
         M, N, K = 1 << mlog, 1 << nlog, 1 << klog
         a = rng.uniform(0.5, 2.0, (nb, M, K)).astype(dt)
         b = rng.uniform(0.5, 2.0, (nb, N, K)).astype(dt)
+        # all-zero slabs (whole 32-deep K slabs of some tiles' rows or columns, and a whole
+        # K range of one batch item), which the split products skip
+        a[:, : M // 2, :32] = 0
+        if K >= 64:
+            b[:, N // 2:, 32:64] = 0
+        if K >= 128:
+            a[1, :, 64:128] = 0
+            b[0, :, 96:128] = 0
         want = np.einsum("bmk,bnk->bmn", a.astype(np.float64), b.astype(np.float64))
         ta = torch.from_numpy(a).to(dev)
         tb = torch.from_numpy(b).to(dev)
